@@ -1,0 +1,408 @@
+"""Python host binding of libfall3d_ads_hip.so (the gfx950 ADS step).
+
+Mirrors the public interface of the reference's MODULE ADS
+(Sources/mod_ADS.F90:48-55) over the C ABI declared in include/fall3d_ads.h:
+
+  ADS.initialize          <- ADS_initialize(limitertype, time_marching)   :98
+  ADS.get_vcomps          <- ADS_get_vcomps(my_u, my_v, my_w)             :166
+  ADS.update_horizontal_vcomps <- ADS_update_horizontal_vcomps(...)       :187
+  ADS.solve_along_x/y/z   <- ADS_solve_along_x/y/z(...)                   :233/:422/:612
+  ADS.solve_1d            <- ADS_solve_1D(...)                            :820
+  ADS.release             <- ADS_release()                                :208
+
+plus the resident F3D_time_step (Sources/mod_F3D.F90:1397-1491) methods
+(set_meteo, set_source, set_state, step, run, time_step, budget).
+
+Arrays are numpy arrays in the reference layouts (Fortran order, halos as
+in the reference; see fall3d_amd/problem.py). There is no CPU fallback:
+constructing an ADS raises if the HIP library is missing or no GPU is
+visible.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfall3d_ads_hip.so")
+
+MINMOD, SUPERBEE, OSPRE = 1, 2, 3
+EULER, RK = 1, 2
+STATE_FP32, STATE_FP32X2 = 0, 1
+LIMITERS = {"minmod": MINMOD, "superbee": SUPERBEE, "ospre": OSPRE}
+TIME_MARCHING = {"euler": EULER, "rk": RK, "runge-kutta": RK}
+STATES = {"fp32": STATE_FP32, "fp32x2": STATE_FP32X2}
+
+BUDGET_MASS_VOLUME, BUDGET_MASS_CELLS, BUDGET_GROUND, BUDGET_LATERAL, BUDGET_INJECTED, BUDGET_LEDGER = 0, 1, 2, 3, 4, 5
+BUDGET_LEN = 11
+KERNELS = {"source": 0, "sweep_x": 1, "sweep_y": 2, "sweep_z": 3, "reduce": 4}
+
+# Every entry point the header declares (checked by the CPU tests).
+EXPORTS = (
+    "fall3d_ads_initialize", "fall3d_ads_set_grid", "fall3d_ads_get_vcomps",
+    "fall3d_ads_update_horizontal_vcomps", "fall3d_ads_solve_along_x", "fall3d_ads_solve_along_y",
+    "fall3d_ads_solve_along_z", "fall3d_ads_solve_1d", "fall3d_ads_release", "fall3d_ads_last_error",
+    "fall3d_ads_set_meteo", "fall3d_ads_set_source", "fall3d_ads_set_state", "fall3d_ads_get_state",
+    "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_run", "fall3d_ads_get_fluxes",
+    "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
+    "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
+)
+
+
+class Fall3dAdsError(RuntimeError):
+    pass
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not found: build it with `python -m fall3d_amd.build`")
+    L = ctypes.CDLL(path)
+    sig = {
+        "fall3d_ads_initialize": ([ctypes.POINTER(_vp), _int, _int, _int, _int, _int, _int, _int, _int], _int),
+        "fall3d_ads_set_grid": ([_vp] * 7, _int),
+        "fall3d_ads_get_vcomps": ([_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _int),
+        "fall3d_ads_update_horizontal_vcomps": ([_vp, _vp, _vp, _vp, _vp, _dbl], _int),
+        "fall3d_ads_solve_along_x": ([_vp, _dp, _dp, _dbl, _vp, _vp], _int),
+        "fall3d_ads_solve_along_y": ([_vp, _dp, _dp, _dbl, _vp, _vp], _int),
+        "fall3d_ads_solve_along_z": ([_vp, _dp, _dp, _dbl, _vp, _vp, _vp, _vp, _vp, _dbl, _vp], _int),
+        "fall3d_ads_solve_1d": ([_vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dp, _dp, ctypes.POINTER(_int), _int, _int,
+                                 _int], _int),
+        "fall3d_ads_release": ([_vp], _int),
+        "fall3d_ads_last_error": ([_vp], ctypes.c_char_p),
+        "fall3d_ads_set_meteo": ([_vp] * 11, _int),
+        "fall3d_ads_set_source": ([_vp, _vp], _int),
+        "fall3d_ads_set_state": ([_vp, _vp, _vp], _int),
+        "fall3d_ads_get_state": ([_vp, _vp, _vp], _int),
+        "fall3d_ads_get_time_step": ([_vp, _dbl, _int, _dbl, _dp], _int),
+        "fall3d_ads_step": ([_vp, _dbl, _dbl, _int], _int),
+        "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
+        "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
+        "fall3d_ads_set_fluxes": ([_vp, _dp], _int),
+        "fall3d_ads_get_budget": ([_vp, _dp], _int),
+        "fall3d_ads_set_stream": ([_vp, _vp], _int),
+        "fall3d_ads_synchronize": ([_vp], _int),
+        "fall3d_ads_set_tiling": ([_vp, _int, _int, _int, _int], _int),
+        "fall3d_ads_set_profiling": ([_vp, _int], _int),
+        "fall3d_ads_get_kernel_time": ([_vp, _int, _dp, ctypes.POINTER(ctypes.c_longlong)], _int),
+        "fall3d_ads_sweep_bytes": ([_vp, _int, _dp], _int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _kind_dtype(kind):
+    return np.float64 if kind == 8 else np.float32
+
+
+class ADS:
+    """One ADS instance (a context of libfall3d_ads_hip.so) for an nx*ny*nz*nbins domain."""
+
+    def __init__(self, nx, ny, nz, nbins, limiter="minmod", time_marching="rk", real_kind=8, state="fp32",
+                 grid=None):
+        L = load_library()
+        self.nx, self.ny, self.nz, self.nb = int(nx), int(ny), int(nz), int(nbins)
+        self.kind = int(real_kind)
+        self.dtype = _kind_dtype(self.kind)
+        lim = LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
+        tm = TIME_MARCHING[time_marching] if isinstance(time_marching, str) else int(time_marching)
+        st = STATES[state] if isinstance(state, str) else int(state)
+        self.limiter, self.time_marching, self.state = lim, tm, st
+        h = _vp()
+        rc = L.fall3d_ads_initialize(ctypes.byref(h), self.nx, self.ny, self.nz, self.nb, lim, tm, self.kind, st)
+        self._h = h
+        if rc != 0:
+            msg = L.fall3d_ads_last_error(h).decode()
+            L.fall3d_ads_release(h)
+            self._h = None
+            raise Fall3dAdsError(f"fall3d_ads_initialize failed ({rc}): {msg}")
+        self._keep = {}
+        if grid is not None:
+            self.set_grid(grid)
+
+    # -- helpers -------------------------------------------------------------
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = load_library().fall3d_ads_last_error(self._h).decode()
+            raise Fall3dAdsError(f"{what} failed ({rc}): {msg}")
+
+    def _arr(self, a, shape, name, writable=False):
+        if a is None:
+            return None
+        a = np.asarray(a)
+        if a.dtype != self.dtype:
+            if writable:
+                raise TypeError(f"{name}: expected dtype {self.dtype}, got {a.dtype}")
+            a = a.astype(self.dtype)
+        if a.size != int(np.prod(shape)):
+            raise ValueError(f"{name}: expected {shape} elements, got shape {a.shape}")
+        if not (a.flags.f_contiguous or a.ndim == 1):
+            if writable:
+                raise ValueError(f"{name}: must be Fortran-contiguous")
+            a = np.asfortranarray(a)
+        return a
+
+    def _p(self, a):
+        return None if a is None else a.ctypes.data
+
+    # -- shapes (reference layouts) -------------------------------------------
+    @property
+    def shape_c(self):
+        return (self.nx + 4, self.ny + 4, self.nz + 4, self.nb)
+
+    @property
+    def shape_acum(self):
+        return (self.nx + 4, self.ny + 4, self.nb)
+
+    @property
+    def shape_u(self):
+        return (self.nx + 3, self.ny, self.nz)
+
+    @property
+    def shape_v(self):
+        return (self.ny + 3, self.nx, self.nz)
+
+    @property
+    def shape_w(self):
+        return (self.nz + 3, self.nx, self.ny)
+
+    @property
+    def shape_vs(self):
+        return (self.nx, self.ny, self.nz + 3, self.nb)
+
+    # -- MODULE ADS --------------------------------------------------------------
+    def set_grid(self, grid):
+        """ARAKAWA_C_GRID spacings dX1_p, dX1_b, dX2_p, dX2_b, dX3_p, dX3_b (dict)."""
+        arrs = [np.ascontiguousarray(grid[k], dtype=self.dtype)
+                for k in ("dX1_p", "dX1_b", "dX2_p", "dX2_b", "dX3_p", "dX3_b")]
+        exp = [self.nx + 4, self.nx + 3, self.ny + 4, self.ny + 3, self.nz + 4, self.nz + 3]
+        for a, n in zip(arrs, exp):
+            if a.size != n:
+                raise ValueError(f"grid array of size {a.size}, expected {n}")
+        self._chk(load_library().fall3d_ads_set_grid(self._h, *[a.ctypes.data for a in arrs]), "set_grid")
+        self._keep["grid"] = arrs
+
+    def get_vcomps(self):
+        """Host arrays CB%u, CB%v, CB%w (writable numpy views owned by the context)."""
+        u, v, w = _vp(), _vp(), _vp()
+        self._chk(load_library().fall3d_ads_get_vcomps(self._h, ctypes.byref(u), ctypes.byref(v), ctypes.byref(w)),
+                  "get_vcomps")
+        ct = ctypes.c_double if self.kind == 8 else ctypes.c_float
+
+        def view(ptr, shape):
+            n = int(np.prod(shape))
+            buf = (ct * n).from_address(ptr.value)
+            return np.frombuffer(buf, dtype=self.dtype).reshape(shape, order="F")
+
+        return view(u, self.shape_u), view(v, self.shape_v), view(w, self.shape_w)
+
+    def update_horizontal_vcomps(self, u1, u2, v1, v2, stime):
+        u1 = self._arr(u1, self.shape_u, "u1"); u2 = self._arr(u2, self.shape_u, "u2")
+        v1 = self._arr(v1, self.shape_v, "v1"); v2 = self._arr(v2, self.shape_v, "v2")
+        self._chk(load_library().fall3d_ads_update_horizontal_vcomps(self._h, u1.ctypes.data, u2.ctypes.data,
+                                                                     v1.ctypes.data, v2.ctypes.data, float(stime)),
+                  "update_horizontal_vcomps")
+
+    def solve_along_x(self, W_flux, E_flux, dt, c, k1):
+        c = self._arr(c, self.shape_c, "c", writable=True)
+        k1 = self._arr(k1, (self.nx + 4, self.ny, self.nz), "k1")
+        W, E = _dbl(W_flux), _dbl(E_flux)
+        self._chk(load_library().fall3d_ads_solve_along_x(self._h, ctypes.byref(W), ctypes.byref(E), float(dt),
+                                                          c.ctypes.data, k1.ctypes.data), "solve_along_x")
+        return W.value, E.value
+
+    def solve_along_y(self, S_flux, N_flux, dt, c, k2):
+        c = self._arr(c, self.shape_c, "c", writable=True)
+        k2 = self._arr(k2, (self.ny + 4, self.nx, self.nz), "k2")
+        S, N = _dbl(S_flux), _dbl(N_flux)
+        self._chk(load_library().fall3d_ads_solve_along_y(self._h, ctypes.byref(S), ctypes.byref(N), float(dt),
+                                                          c.ctypes.data, k2.ctypes.data), "solve_along_y")
+        return S.value, N.value
+
+    def solve_along_z(self, D_flux, U_flux, dt, c, acum, k3, w1, w2, stime, vs):
+        c = self._arr(c, self.shape_c, "c", writable=True)
+        acum = self._arr(acum, self.shape_acum, "acum", writable=True)
+        k3 = self._arr(k3, (self.nz + 4, self.nx, self.ny), "k3")
+        w1 = self._arr(w1, self.shape_w, "w1")
+        w2 = self._arr(w2, self.shape_w, "w2")
+        vs = self._arr(vs, self.shape_vs, "vs")
+        D, U = _dbl(D_flux), _dbl(U_flux)
+        self._chk(load_library().fall3d_ads_solve_along_z(self._h, ctypes.byref(D), ctypes.byref(U), float(dt),
+                                                          c.ctypes.data, acum.ctypes.data, k3.ctypes.data,
+                                                          w1.ctypes.data, w2.ctypes.data, float(stime),
+                                                          vs.ctypes.data), "solve_along_z")
+        return D.value, U.value
+
+    def solve_1d(self, c, dxp, dxb, u, k, dt, bvalue, bcode, ips, ipe, limiter):
+        """ADS_solve_1D on one line; c is updated in place, returns flux(2)."""
+        n = ipe - ips + 1
+        c = self._arr(c, (n + 4,), "c", writable=True)
+        dxp = self._arr(dxp, (n + 4,), "dxp"); k = self._arr(k, (n + 4,), "k")
+        dxb = self._arr(dxb, (n + 3,), "dxb"); u = self._arr(u, (n + 3,), "u")
+        bv = (ctypes.c_double * 2)(*bvalue)
+        fl = (ctypes.c_double * 2)()
+        bc = (ctypes.c_int * 2)(*bcode)
+        self._chk(load_library().fall3d_ads_solve_1d(self._h, c.ctypes.data, dxp.ctypes.data, dxb.ctypes.data,
+                                                     u.ctypes.data, k.ctypes.data, float(dt), bv, fl, bc, int(ips),
+                                                     int(ipe), int(limiter)), "solve_1d")
+        return fl[0], fl[1]
+
+    def release(self):
+        if getattr(self, "_h", None):
+            load_library().fall3d_ads_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    # -- resident F3D_time_step --------------------------------------------------
+    def set_meteo(self, u1, v1, w1, k1, k2, k3, vs, u2=None, v2=None, w2=None):
+        a = {
+            "u1": self._arr(u1, self.shape_u, "u1"), "u2": self._arr(u2, self.shape_u, "u2"),
+            "v1": self._arr(v1, self.shape_v, "v1"), "v2": self._arr(v2, self.shape_v, "v2"),
+            "w1": self._arr(w1, self.shape_w, "w1"), "w2": self._arr(w2, self.shape_w, "w2"),
+            "k1": self._arr(k1, (self.nx + 4, self.ny, self.nz), "k1"),
+            "k2": self._arr(k2, (self.ny + 4, self.nx, self.nz), "k2"),
+            "k3": self._arr(k3, (self.nz + 4, self.nx, self.ny), "k3"),
+            "vs": self._arr(vs, self.shape_vs, "vs"),
+        }
+        p = self._p
+        self._chk(load_library().fall3d_ads_set_meteo(self._h, p(a["u1"]), p(a["u2"]), p(a["v1"]), p(a["v2"]),
+                                                      p(a["w1"]), p(a["w2"]), p(a["k1"]), p(a["k2"]), p(a["k3"]),
+                                                      p(a["vs"])), "set_meteo")
+
+    def set_problem(self, prob):
+        """Grid, meteo, source and initial state of a fall3d_amd.problem.Problem."""
+        self.set_grid(prob.grid)
+        self.set_meteo(prob.u1, prob.v1, prob.w1, prob.k1, prob.k2, prob.k3, prob.vs, prob.u2, prob.v2, prob.w2)
+        self.set_source(prob.s)
+        self.set_state(prob.c0)
+
+    def set_source(self, s):
+        s = self._arr(s, (self.nx, self.ny, self.nz, self.nb), "s") if s is not None else None
+        self._chk(load_library().fall3d_ads_set_source(self._h, self._p(s)), "set_source")
+
+    def set_state(self, c, acum=None):
+        c = self._arr(c, self.shape_c, "c")
+        acum = self._arr(acum, self.shape_acum, "acum") if acum is not None else None
+        self._chk(load_library().fall3d_ads_set_state(self._h, c.ctypes.data, self._p(acum)), "set_state")
+
+    def get_state(self):
+        c = np.zeros(self.shape_c, dtype=self.dtype, order="F")
+        acum = np.zeros(self.shape_acum, dtype=self.dtype, order="F")
+        self._chk(load_library().fall3d_ads_get_state(self._h, c.ctypes.data, acum.ctypes.data), "get_state")
+        return c, acum
+
+    def time_step(self, stime=0.0, criterion=2, cfl=0.9):
+        dt = _dbl()
+        self._chk(load_library().fall3d_ads_get_time_step(self._h, float(stime), int(criterion), float(cfl),
+                                                          ctypes.byref(dt)), "get_time_step")
+        return dt.value
+
+    def step(self, dt, iiter, stime=0.0):
+        self._chk(load_library().fall3d_ads_step(self._h, float(dt), float(stime), int(iiter)), "step")
+
+    def run(self, dt, nsteps, iiter0=0, stime=0.0):
+        self._chk(load_library().fall3d_ads_run(self._h, float(dt), float(stime), int(iiter0), int(nsteps)), "run")
+
+    def synchronize(self):
+        self._chk(load_library().fall3d_ads_synchronize(self._h), "synchronize")
+
+    def fluxes(self):
+        f = (ctypes.c_double * 6)()
+        self._chk(load_library().fall3d_ads_get_fluxes(self._h, f), "get_fluxes")
+        return np.array(f[:])
+
+    def set_fluxes(self, flux):
+        f = (ctypes.c_double * 6)(*flux)
+        self._chk(load_library().fall3d_ads_set_fluxes(self._h, f), "set_fluxes")
+
+    def budget(self):
+        b = (ctypes.c_double * BUDGET_LEN)()
+        self._chk(load_library().fall3d_ads_get_budget(self._h, b), "get_budget")
+        out = np.array(b[:])
+        return {
+            "mass_volume": out[0], "mass_cells": out[1], "ground": out[2], "lateral": out[3], "injected": out[4],
+            "ledger": out[5:11].copy(),
+        }
+
+    def set_stream(self, stream_handle):
+        self._chk(load_library().fall3d_ads_set_stream(self._h, stream_handle), "set_stream")
+
+    def set_tiling(self, axis, cells_per_thread=0, columns=0, bins_per_group=0):
+        self._chk(load_library().fall3d_ads_set_tiling(self._h, int(axis), int(cells_per_thread), int(columns),
+                                                       int(bins_per_group)), "set_tiling")
+
+    def set_profiling(self, on=True):
+        self._chk(load_library().fall3d_ads_set_profiling(self._h, 1 if on else 0), "set_profiling")
+
+    def kernel_time(self, kernel):
+        ms, n = _dbl(), ctypes.c_longlong()
+        kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        self._chk(load_library().fall3d_ads_get_kernel_time(self._h, kid, ctypes.byref(ms), ctypes.byref(n)),
+                  "get_kernel_time")
+        return ms.value, n.value
+
+    def sweep_bytes(self, axis):
+        b = _dbl()
+        self._chk(load_library().fall3d_ads_sweep_bytes(self._h, int(axis), ctypes.byref(b)), "sweep_bytes")
+        return b.value
+
+
+# ---- module-level singleton, as the reference's private CB (mod_ADS.F90:41) ----
+_default: ADS | None = None
+
+
+def ADS_initialize(limitertype, time_marching, nx, ny, nz, nbins, grid, real_kind=8, state="fp32"):
+    """ADS_initialize with the Domain extents and the grid passed explicitly."""
+    global _default
+    if _default is not None:
+        _default.release()
+    _default = ADS(nx, ny, nz, nbins, limitertype, time_marching, real_kind, state, grid=grid)
+    return _default
+
+
+def ADS_get_vcomps():
+    return _default.get_vcomps()
+
+
+def ADS_update_horizontal_vcomps(my_u1, my_u2, my_v1, my_v2, stime):
+    _default.update_horizontal_vcomps(my_u1, my_u2, my_v1, my_v2, stime)
+
+
+def ADS_solve_along_x(my_W_flux, my_E_flux, dt, my_c, my_k1):
+    return _default.solve_along_x(my_W_flux, my_E_flux, dt, my_c, my_k1)
+
+
+def ADS_solve_along_y(my_S_flux, my_N_flux, dt, my_c, my_k2):
+    return _default.solve_along_y(my_S_flux, my_N_flux, dt, my_c, my_k2)
+
+
+def ADS_solve_along_z(my_D_flux, my_U_flux, dt, my_c, my_acum, my_k3, my_w1, my_w2, stime, my_vs):
+    return _default.solve_along_z(my_D_flux, my_U_flux, dt, my_c, my_acum, my_k3, my_w1, my_w2, stime, my_vs)
+
+
+def ADS_release():
+    global _default
+    if _default is not None:
+        _default.release()
+        _default = None

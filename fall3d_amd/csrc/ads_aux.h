@@ -1,0 +1,52 @@
+// ads_aux.h -- launchers of the kernels in ads_aux.hip (internal).
+#pragma once
+
+#include "ads_internal.h"
+
+namespace f3d {
+
+// Up to 16 reductions of sweep partials (4 doubles per block) into acc[].
+struct ReduceJob {
+  const double *part[16];
+  int nblocks[16];
+  int slot[16];
+  int dst[16];
+  int count;
+};
+
+struct CflArgs {
+  int nx, ny, nz, nb, criterion;
+  double stime, cfl;
+  const float *u1, *u2, *v1, *v2, *w1, *w2, *k1, *k2, *k3, *vs;
+  const double *dx1p, *dx2p, *dx3p;  // FP64 dX*_p, index p+1 for p = -1..n+2
+};
+
+struct Solve1DArgs {
+  double *c;          // n+4, positions ips-2..ipe+2
+  const double *dxp;  // n+4
+  const double *dxb;  // n+3, positions ibs-1..ibe+1
+  const double *u;    // n+3
+  const double *k;    // n+4
+  double *flux;       // 2
+  double dt;
+  double bvalue[2];
+  int bcode[2];
+  int n, limiter;
+};
+
+hipError_t state_in(const void *src, int kind, float *hi, float *lo, size_t n, hipStream_t st);
+hipError_t state_out(const float *hi, const float *lo, void *dst, int kind, size_t n, hipStream_t st);
+hipError_t to_f32(const void *src, int kind, float *dst, size_t n, hipStream_t st);
+hipError_t to_f64(const void *src, int kind, double *dst, size_t n, hipStream_t st);
+hipError_t transpose_to_f32(const void *src, int kind, float *dst, int A, int B, int K, hipStream_t st);
+hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, hipStream_t st);
+hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double *rate_vol, const double *meas,
+                         int nent, double dt, double *partials, int *nblocks, hipStream_t st);
+hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st);
+hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *out, hipStream_t st);
+hipError_t mass(const float *chi, const float *clo, int nx, int ny, int nz, int nb, const double *d1,
+                const double *d2, const double *d3, int off, double *part, int part_cap, double *out,
+                hipStream_t st);
+hipError_t solve_1d(const Solve1DArgs &a, hipStream_t st);
+
+}  // namespace f3d

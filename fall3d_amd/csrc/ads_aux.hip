@@ -1,0 +1,477 @@
+// ads_aux.hip -- the kernels around the sweeps: host-kind conversion and
+// layout transposes (meteo upload), sparse source injection, deterministic
+// FP64 reductions of the boundary sums, CFL time step, mass budget and the
+// single-line ADS_solve_1D.
+#include "ads_aux.h"
+
+namespace f3d {
+
+// ---------------------------------------------------------------------------
+// conversion / transposes
+// ---------------------------------------------------------------------------
+
+// host-kind state -> (hi, lo) FP32 pair; lo carries the rounding residual of
+// an FP64 input so the compensated state starts from ~48 significant bits.
+template <typename T>
+__global__ void k_state_in(const T *__restrict__ src, float *__restrict__ hi, float *__restrict__ lo, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double x = (double)src[i];
+    const float h = (float)x;
+    hi[i] = h;
+    if (lo) lo[i] = (float)(x - (double)h);
+  }
+}
+
+template <typename T>
+__global__ void k_state_out(const float *__restrict__ hi, const float *__restrict__ lo, T *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double x = (double)hi[i];
+    if (lo) x += (double)lo[i];
+    dst[i] = (T)x;
+  }
+}
+
+template <typename T>
+__global__ void k_to_f32(const T *__restrict__ src, float *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+template <typename T>
+__global__ void k_to_f64(const T *__restrict__ src, double *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (double)src[i];
+}
+
+// dst[b + B*(a + A*k)] = src[a + A*(b + B*k)]  (swap the two fastest axes),
+// 32x32 tiles through LDS so both sides are coalesced.
+template <typename T>
+__global__ void k_transpose(const T *__restrict__ src, float *__restrict__ dst, int A, int B, int K) {
+  __shared__ float tile[32][33];
+  const int a0 = blockIdx.x * 32, b0 = blockIdx.y * 32;
+  const int k = blockIdx.z;
+  const size_t off = (size_t)A * B * k;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int a = a0 + threadIdx.x, b = b0 + r;
+    if (a < A && b < B) tile[r][threadIdx.x] = (float)src[off + a + (size_t)A * b];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int b = b0 + threadIdx.x, a = a0 + r;
+    if (a < A && b < B) dst[off + b + (size_t)B * a] = tile[threadIdx.x][r];
+  }
+}
+
+__global__ void k_lerp(const float *__restrict__ a, const float *__restrict__ b, float *__restrict__ out, float s,
+                       size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (1.0f - s) * a[i] + s * b[i];
+}
+
+static inline unsigned grid1d(size_t n, int bs = 256) {
+  size_t g = (n + bs - 1) / bs;
+  return (unsigned)(g > 65536 ? 65536 : (g == 0 ? 1 : g));
+}
+
+hipError_t state_in(const void *src, int kind, float *hi, float *lo, size_t n, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  if (kind == 8)
+    hipLaunchKernelGGL(k_state_in<double>, dim3(grid1d(n)), dim3(256), 0, st, (const double *)src, hi, lo, n);
+  else
+    hipLaunchKernelGGL(k_state_in<float>, dim3(grid1d(n)), dim3(256), 0, st, (const float *)src, hi, lo, n);
+  return hipGetLastError();
+}
+
+hipError_t state_out(const float *hi, const float *lo, void *dst, int kind, size_t n, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  if (kind == 8)
+    hipLaunchKernelGGL(k_state_out<double>, dim3(grid1d(n)), dim3(256), 0, st, hi, lo, (double *)dst, n);
+  else
+    hipLaunchKernelGGL(k_state_out<float>, dim3(grid1d(n)), dim3(256), 0, st, hi, lo, (float *)dst, n);
+  return hipGetLastError();
+}
+
+hipError_t to_f32(const void *src, int kind, float *dst, size_t n, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  if (kind == 8)
+    hipLaunchKernelGGL(k_to_f32<double>, dim3(grid1d(n)), dim3(256), 0, st, (const double *)src, dst, n);
+  else
+    hipLaunchKernelGGL(k_to_f32<float>, dim3(grid1d(n)), dim3(256), 0, st, (const float *)src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t to_f64(const void *src, int kind, double *dst, size_t n, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  if (kind == 8)
+    hipLaunchKernelGGL(k_to_f64<double>, dim3(grid1d(n)), dim3(256), 0, st, (const double *)src, dst, n);
+  else
+    hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1d(n)), dim3(256), 0, st, (const float *)src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t transpose_to_f32(const void *src, int kind, float *dst, int A, int B, int K, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  dim3 grid((A + 31) / 32, (B + 31) / 32, K), block(32, 8);
+  if (kind == 8)
+    hipLaunchKernelGGL(k_transpose<double>, grid, block, 0, st, (const double *)src, dst, A, B, K);
+  else if (kind == 4)
+    hipLaunchKernelGGL(k_transpose<float>, grid, block, 0, st, (const float *)src, dst, A, B, K);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  hipLaunchKernelGGL(k_lerp, dim3(grid1d(n)), dim3(256), 0, st, a, b, out, s, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// source injection (Sources/mod_F3D.F90:1456-1473), sparse entries
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void comp_add2(float &hi, float &lo, float t) {
+  const float s = hi + t;
+  const float bp = s - hi;
+  const float e = (hi - (s - bp)) + (t - bp);
+  const float l = lo + e;
+  const float h2 = s + l;
+  lo = l - (h2 - s);
+  hi = h2;
+}
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Block sum of one double per thread into out[blockIdx.x] (fixed order).
+__device__ void block_sum_store(double v, double *out) {
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  v = wsum(v);
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += red[w];
+    out[blockIdx.x] = s;
+  }
+}
+
+// Each entry: c[idx] += dt * rate_vol (rate_vol = s/vol in FP64); the
+// injected mass (increment times the cell's mass measure) is summed into
+// partials[blockIdx.x*4 + 0] (slots 1..3 zero, same layout as the sweeps).
+__global__ void k_source(float *__restrict__ chi, float *__restrict__ clo, const size_t *__restrict__ idx,
+                         const double *__restrict__ rate_vol, const double *__restrict__ meas, int nent, double dt,
+                         double *__restrict__ partials) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (e < nent) {
+    const double inc = dt * rate_vol[e];
+    const size_t i = idx[e];
+    if (clo) {
+      const float ih = (float)inc;
+      const float il = (float)(inc - (double)ih);
+      float h = chi[i], l = clo[i];
+      comp_add2(h, l, ih);
+      comp_add2(h, l, il);
+      chi[i] = h;
+      clo[i] = l;
+      m = inc * meas[e];
+    } else {
+      const float old = chi[i];
+      const float nv = old + (float)inc;
+      chi[i] = nv;
+      m = ((double)nv - (double)old) * meas[e];
+    }
+  }
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  m = wsum(m);
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+      const int nw = (blockDim.x + 63) >> 6;
+      for (int w = 0; w < nw; ++w) s += red[w];
+    }
+    partials[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
+  }
+}
+
+hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double *rate_vol, const double *meas,
+                         int nent, double dt, double *partials, int *nblocks, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  const int nb = (nent + 255) / 256;
+  *nblocks = nb;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_source, dim3(nb), dim3(256), 0, st, chi, clo, idx, rate_vol, meas, nent, dt, partials);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reduction of sweep partials into the accumulators
+// ---------------------------------------------------------------------------
+
+// One wave per (array, slot): sums partials[b*4+slot] over b in a fixed
+// lane-strided order, then a fixed shuffle tree; adds into acc[dst].
+__global__ void k_finalize(double *__restrict__ acc, ReduceJob jobs) {
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wid >= jobs.count) return;
+  const double *p = jobs.part[wid];
+  const int nb = jobs.nblocks[wid];
+  const int slot = jobs.slot[wid];
+  double s = 0.0;
+  for (int b = lane; b < nb; b += 64) s += p[(size_t)b * 4 + slot];
+  s = wsum(s);
+  if (lane == 0) acc[jobs.dst[wid]] += s;
+}
+
+hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  if (jobs.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64 * jobs.count), 0, st, acc, jobs);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// CFL time step, Sources/mod_Grid.F90:1026-1141 (FP64 arithmetic and max)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wmax(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+__global__ void k_cfl(CflArgs a, double *__restrict__ part) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const size_t ncell = (size_t)nx * ny * nz;
+  double m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  const double s = a.stime, s1 = 1.0 - a.stime;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx) + 1;
+    const int j = (int)((c / nx) % ny) + 1;
+    const int k = (int)(c / ((size_t)nx * ny)) + 1;
+    const double dx = a.dx1p[i + 1], dy = a.dx2p[j + 1], dz = a.dx3p[k + 1];
+    const size_t iu = (size_t)i + (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * (k - 1));
+    const size_t iv = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * (k - 1));
+    const size_t iw = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * k);
+    const double u = s1 * (double)a.u1[iu] + s * (double)a.u2[iu];
+    const double v = s1 * (double)a.v1[iv] + s * (double)a.v2[iv];
+    const double w = s1 * (double)a.w1[iw] + s * (double)a.w2[iw];
+    const double k1 = a.k1[(size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j - 1) + (size_t)ny * (k - 1))];
+    const double k2 = a.k2[(size_t)(i - 1) + (size_t)nx * ((size_t)(j + 1) + (size_t)(ny + 4) * (k - 1))];
+    const double k3 = a.k3[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k + 1))];
+    double wm = 0.0;
+    for (int b = 0; b < a.nb; ++b) {
+      const double vs = a.vs[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)k + (size_t)(nz + 3) * b))];
+      wm = fmax(wm, fabs(w - vs));
+    }
+    const double t1 = 2.0 * k1 / (dx * dx) + fabs(u) / dx;
+    const double t2 = 2.0 * k2 / (dy * dy) + fabs(v) / dy;
+    const double t3 = 2.0 * k3 / (dz * dz) + wm / dz;
+    if (a.criterion == 1) {
+      m1 = fmax(m1, t1);
+      m2 = fmax(m2, t2);
+      m3 = fmax(m3, t3);
+    } else {
+      m1 = fmax(m1, t1 + t2 + t3);
+    }
+  }
+  __shared__ double red[3][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  m1 = wmax(m1);
+  m2 = wmax(m2);
+  m3 = wmax(m3);
+  if (lane == 0) { red[0][wid] = m1; red[1][wid] = m2; red[2][wid] = m3; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) { r1 = fmax(r1, red[0][w]); r2 = fmax(r2, red[1][w]); r3 = fmax(r3, red[2][w]); }
+    part[blockIdx.x] = fmax(fmax(r1, r2), r3);
+  }
+}
+
+__global__ void k_cfl_final(const double *__restrict__ part, int nb, double cfl, double *__restrict__ out) {
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) m = fmax(m, part[b]);
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  m = wmax(m);
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) r = fmax(r, red[w]);
+    double dt = r;
+    if (dt > 0.0) dt = 1.0 / dt;
+    out[0] = cfl * dt;
+  }
+}
+
+hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *out, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  const size_t ncell = (size_t)a.nx * a.ny * a.nz;
+  int nb = (int)((ncell + 255) / 256);
+  if (nb > part_cap) nb = part_cap;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, a, part);
+  hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, st, part, nb, a.cfl, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// mass in the volume, Sources/mod_Grid.F90:1364-1375 (FP64)
+// ---------------------------------------------------------------------------
+
+__global__ void k_mass(const float *__restrict__ chi, const float *__restrict__ clo, int nx, int ny, int nz, int nb,
+                       const double *__restrict__ d1, const double *__restrict__ d2, const double *__restrict__ d3,
+                       int off, double *__restrict__ part) {
+  const size_t ncell = (size_t)nx * ny * nz * nb;
+  double m = 0.0;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx) + 1;
+    size_t r = c / nx;
+    const int j = (int)(r % ny) + 1;
+    r /= ny;
+    const int k = (int)(r % nz) + 1;
+    const int b = (int)(r / nz);
+    const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
+    double cv = (double)chi[ci];
+    if (clo) cv += (double)clo[ci];
+    m += cv * d1[i + off] * d2[j + off] * d3[k + off];
+  }
+  block_sum_store(m, part);
+}
+
+__global__ void k_sum_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  s = wsum(s);
+  if (lane == 0) red[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) r += red[w];
+    out[0] = r;
+  }
+}
+
+hipError_t mass(const float *chi, const float *clo, int nx, int ny, int nz, int nb, const double *d1,
+                const double *d2, const double *d3, int off, double *part, int part_cap, double *out,
+                hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  const size_t n = (size_t)nx * ny * nz * nb;
+  int g = (int)((n + 255) / 256);
+  if (g > part_cap) g = part_cap;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_mass, dim3(g), dim3(256), 0, st, chi, clo, nx, ny, nz, nb, d1, d2, d3, off, part);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, part, g, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// ADS_solve_1D (Sources/mod_ADS.F90:820-1053), one line in FP64, one workgroup
+// ---------------------------------------------------------------------------
+
+__device__ double lim1d(int lim, double cm, double c0, double cp) {
+  const double eps = 1e-12;
+  if (fabs(cp - c0) < eps) {
+    if (lim == 1) return 1.0;
+    if (fabs(c0 - cm) < eps) return 0.0;
+    return lim == 2 ? 2.0 : 1.5;
+  }
+  const double r = (c0 - cm) / (cp - c0);
+  if (lim == 1) return fmax(0.0, fmin(1.0, r));
+  if (lim == 2) return fmax(fmax(0.0, fmin(2.0 * r, 1.0)), fmin(r, 2.0));
+  return 1.5 * (r * r + r) / (r * r + r + 1.0);
+}
+
+// line arrays are indexed from the line's first halo element (position ips-2)
+__global__ void k_solve_1d(Solve1DArgs a) {
+  extern __shared__ double sh[];
+  const int n = a.n;                 // interior cells
+  double *c = sh;                    // n+4: positions -1..n+2 -> [p+1]
+  double *c0 = c + (n + 4);          // n+4
+  double *r = c0 + (n + 4);          // n+4
+  double *kk = r + (n + 4);          // 4 stages x n
+  double *flux = kk + 4 * n;         // 2
+  const int t = threadIdx.x;
+  for (int p = t; p < n + 4; p += blockDim.x) c[p] = a.c[p];
+  __syncthreads();
+  if (t == 0) {
+    // boundary conditions, Sources/mod_ADS.F90:907-983
+    const double ub = a.u[1], ue = a.u[n + 1];
+    if (a.bcode[0] == -1) {
+      if (ub > 0.0) { c[1] = c[n + 1]; c[0] = c[n]; c[n + 2] = c[2]; c[n + 3] = c[3]; }
+    } else if (a.bcode[0] == 1) {
+      c[1] = 2.0 * a.bvalue[0] - c[2]; c[0] = c[1];
+    } else if (a.bcode[0] == 2) {
+      if (ub > 0.0) { c[1] = 2.0 * a.bvalue[0] - c[2]; c[0] = c[1]; }
+      else { c[1] = c[2]; c[0] = c[1]; }
+    }
+    if (a.bcode[1] == -1) {
+      if (ue < 0.0) { c[1] = c[n + 1]; c[0] = c[n]; c[n + 2] = c[2]; c[n + 3] = c[3]; }
+    } else if (a.bcode[1] == 1) {
+      c[n + 2] = 2.0 * a.bvalue[1] - c[n + 1]; c[n + 3] = c[n + 2];
+    } else if (a.bcode[1] == 2) {
+      if (ue < 0.0) { c[n + 2] = 2.0 * a.bvalue[1] - c[n + 1]; c[n + 3] = c[n + 2]; }
+      else { c[n + 2] = c[n + 1]; c[n + 3] = c[n + 2]; }
+    }
+  }
+  __syncthreads();
+  for (int p = t; p < n + 4; p += blockDim.x) c0[p] = c[p];
+  __syncthreads();
+  for (int st = 0; st < 4; ++st) {
+    // limiter at positions 0..n+1 (array index p+1)
+    for (int p = t; p <= n + 1; p += blockDim.x) r[p + 1] = lim1d(a.limiter, c0[p], c0[p + 1], c0[p + 2]);
+    __syncthreads();
+    if (t == 0) { r[0] = r[1]; r[n + 3] = r[n + 2]; }
+    __syncthreads();
+    for (int i = 1 + t; i <= n; i += blockDim.x) {
+      // c_r(i), c_l(i), c_r(i+1), c_l(i+1) (Sources/mod_ADS.F90:1094-1097)
+      const double cr_i = c0[i + 1] - 0.5 * r[i + 1] * (c0[i + 2] - c0[i + 1]);
+      const double cl_i = c0[i] + 0.5 * r[i] * (c0[i + 1] - c0[i]);
+      const double cr_p = c0[i + 2] - 0.5 * r[i + 2] * (c0[i + 3] - c0[i + 2]);
+      const double cl_p = c0[i + 1] + 0.5 * r[i + 1] * (c0[i + 2] - c0[i + 1]);
+      const double ui = a.u[i], up = a.u[i + 1];
+      const double F_m = 0.5 * (cl_i * ui + cr_i * ui) - 0.5 * fabs(ui) * (cr_i - cl_i);
+      const double F_p = 0.5 * (cl_p * up + cr_p * up) - 0.5 * fabs(up) * (cr_p - cl_p);
+      const double P_m = 0.5 * (a.k[i + 1] + a.k[i]) * (c0[i + 1] - c0[i]) / a.dxp[i];
+      const double P_p = 0.5 * (a.k[i + 2] + a.k[i + 1]) * (c0[i + 2] - c0[i + 1]) / a.dxp[i + 1];
+      kk[st * n + i - 1] = -1.0 * (F_p - F_m) / a.dxb[i] + (P_p - P_m) / a.dxb[i];
+      if (i == 1) flux[0] = F_m + P_m;
+      if (i == n) flux[1] = F_p + P_p;
+    }
+    __syncthreads();
+    if (st < 3) {
+      const double h = (st == 2) ? a.dt : 0.5 * a.dt;
+      for (int i = 1 + t; i <= n; i += blockDim.x) c0[i + 1] = c[i + 1] + h * kk[st * n + i - 1];
+      __syncthreads();
+    }
+  }
+  for (int i = 1 + t; i <= n; i += blockDim.x) {
+    const int q = i - 1;
+    c[i + 1] = c[i + 1] + a.dt * (kk[q] + 2.0 * kk[n + q] + 2.0 * kk[2 * n + q] + kk[3 * n + q]) / 6.0;
+  }
+  __syncthreads();
+  for (int p = t; p < n + 4; p += blockDim.x) a.c[p] = c[p];
+  if (t == 0) { a.flux[0] = flux[0]; a.flux[1] = flux[1]; }
+}
+
+hipError_t solve_1d(const Solve1DArgs &a, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  const size_t shmem = sizeof(double) * (size_t)(3 * (a.n + 4) + 4 * a.n + 2);
+  if (shmem > 60000) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_solve_1d, dim3(1), dim3(256), shmem, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace f3d

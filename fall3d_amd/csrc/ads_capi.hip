@@ -1,0 +1,1055 @@
+// ads_capi.hip -- the extern "C" ABI of libfall3d_ads_hip.so (include/fall3d_ads.h).
+//
+// Host-side orchestration of the gfx950 kernels: context and device memory,
+// host<->device staging for the drop-in MODULE ADS entry points, the
+// resident F3D_time_step, reductions, CFL, budget and per-kernel timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fall3d_ads.h"
+#include "ads_aux.h"
+#include "ads_internal.h"
+
+using namespace f3d;
+
+namespace {
+
+enum { KID_SOURCE = 0, KID_SWEEP_X = 1, KID_SWEEP_Y = 2, KID_SWEEP_Z = 3, KID_REDUCE = 4, KID_COUNT = 5 };
+
+// device accumulator layout
+enum { ACC_FLUX = 0, ACC_LEDGER = 6, ACC_INJECTED = 12, ACC_SCRATCH = 13, ACC_LEN = 16 };
+
+struct Tiling {
+  int cpt = 0;      // cells per thread
+  int S = 0;        // segments per line
+  int TX = 0;       // lines per workgroup (y, z) / lines per workgroup (x)
+  int block = 0;    // threads per workgroup
+  int gx = 0;       // workgroups along grid.x
+  int groups = 1;   // bin groups (grid.y)
+  int bpg = 1;      // bins per group
+  int ntrans = 0;   // x tiles (y, z sweeps)
+};
+
+struct PendingEvent {
+  int kid;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct fall3d_ads_ctx {
+  int nx = 0, ny = 0, nz = 0, nb = 0;
+  int limiter = 1, tm = 2, kind = 8, comp = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err = "no error";
+
+  // state
+  float *chi = nullptr, *clo = nullptr;
+  double *acum = nullptr;
+  size_t ncell_c = 0;  // (nx+4)(ny+4)(nz+4)nb
+  size_t nacum = 0;    // (nx+4)(ny+4)nb
+
+  // meteo (device layouts, FP32)
+  float *u1 = nullptr, *u2 = nullptr, *v1 = nullptr, *v2 = nullptr, *w1 = nullptr, *w2 = nullptr;
+  float *k1 = nullptr, *k2 = nullptr, *k3 = nullptr, *vs = nullptr;
+  bool meteo_ok = false;
+  bool has_w2 = false;
+
+  // grid
+  double *dxp64[3] = {nullptr, nullptr, nullptr};
+  double *dxb64[3] = {nullptr, nullptr, nullptr};
+  float *hdxp[3] = {nullptr, nullptr, nullptr};
+  float *idxb[3] = {nullptr, nullptr, nullptr};
+  std::vector<double> hdxp64[3], hdxb64[3];
+  bool grid_ok = false;
+
+  // sparse source
+  int nsrc = 0;
+  size_t *src_idx = nullptr;
+  double *src_rate = nullptr, *src_meas = nullptr;
+  double *src_part = nullptr;
+
+  // accumulators and reduction scratch
+  double *acc = nullptr;
+  double *part[4] = {nullptr, nullptr, nullptr, nullptr};  // source, x, y, z
+  size_t part_cap[4] = {0, 0, 0, 0};                        // in blocks
+  double *red_part = nullptr;
+  int red_cap = 0;
+
+  // staging
+  void *dstage = nullptr;
+  size_t dstage_bytes = 0;
+  void *hu = nullptr, *hv = nullptr, *hw = nullptr;  // CB%u, CB%v, CB%w host arrays
+
+  // tiling
+  Tiling tl[3];
+  int user_cpt[3] = {0, 0, 0}, user_tx[3] = {0, 0, 0}, user_bpg[3] = {0, 0, 0};
+
+  // profiling
+  bool prof = false;
+  std::vector<PendingEvent> pending;
+  std::vector<hipEvent_t> evpool;
+  double kt_ms[KID_COUNT] = {0, 0, 0, 0, 0};
+  long long kt_n[KID_COUNT] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(fall3d_ads_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return fail((ctx), FALL3D_ADS_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void **)p, n * sizeof(T));
+}
+
+void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+int axis_n(const fall3d_ads_ctx *c, int dir) { return dir == 0 ? c->nx : (dir == 1 ? c->ny : c->nz); }
+
+int ensure_stage(fall3d_ads_ctx *c, size_t bytes) {
+  if (c->dstage_bytes >= bytes) return FALL3D_ADS_OK;
+  dfree(c->dstage);
+  c->dstage = nullptr;
+  c->dstage_bytes = 0;
+  HIPCHK(c, hipMalloc(&c->dstage, bytes));
+  c->dstage_bytes = bytes;
+  return FALL3D_ADS_OK;
+}
+
+// ---- profiling -------------------------------------------------------------
+hipEvent_t get_event(fall3d_ads_ctx *c) {
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct ProfScope {
+  fall3d_ads_ctx *c;
+  int kid;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(fall3d_ads_ctx *ctx, int k) : c(ctx), kid(k) {
+    if (c->prof) {
+      a = get_event(c);
+      b = get_event(c);
+      if (a) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->prof && a && b) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({kid, a, b});
+    }
+  }
+};
+
+void drain_events(fall3d_ads_ctx *c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->kt_ms[p.kid] += ms;
+      c->kt_n[p.kid] += 1;
+    }
+    c->evpool.push_back(p.a);
+    c->evpool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+// ---- tiling -------------------------------------------------------------------
+const int kCpt[] = {2, 3, 4, 6, 8, 12, 16};
+
+Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
+  Tiling t;
+  const int n = axis_n(c, dir);
+  const int ucpt = c->user_cpt[dir], utx = c->user_tx[dir];
+  if (dir == 0) {
+    int C = 16;
+    if (ucpt) {
+      C = ucpt;
+    } else {
+      for (int cc : kCpt) {
+        if ((n + cc - 1) / cc <= 64) { C = cc; break; }
+      }
+    }
+    t.cpt = C;
+    t.S = (n + C - 1) / C;
+    t.block = 256;
+    if (t.S > 256) t.S = 0;  // unsupported; caught by the caller
+    const int lpb = t.S ? 256 / t.S : 1;
+    t.TX = lpb;
+    const long nlines = (long)c->ny * c->nz;
+    t.gx = (int)((nlines + lpb - 1) / lpb);
+    t.ntrans = 0;
+  } else {
+    int bestC = 0, bestTX = 0;
+    if (ucpt || utx) {
+      bestC = ucpt ? ucpt : 8;
+      bestTX = utx ? utx : 64;
+    } else {
+      const int txs[3] = {64, 32, 16};
+      for (int tx : txs) {
+        for (int cc : {2, 3, 4, 6, 8}) {
+          if (((n + cc - 1) / cc) * tx <= 512) { bestC = cc; bestTX = tx; break; }
+        }
+        if (bestC) break;
+      }
+      if (!bestC) {
+        const int txs2[4] = {32, 16, 8, 4};
+        for (int tx : txs2) {
+          for (int cc : {12, 16}) {
+            if (((n + cc - 1) / cc) * tx <= 512) { bestC = cc; bestTX = tx; break; }
+          }
+          if (bestC) break;
+        }
+      }
+    }
+    t.cpt = bestC;
+    t.TX = bestTX;
+    t.S = bestC ? (n + bestC - 1) / bestC : 0;
+    t.block = t.S * t.TX;
+    if (t.block > 512 || t.block <= 0) t.S = 0;
+    t.ntrans = (c->nx + bestTX - 1) / (bestTX ? bestTX : 1);
+    t.gx = t.ntrans * (dir == 1 ? c->nz : c->ny);
+  }
+  // bins per workgroup: all bins (coefficients read once) unless the grid
+  // would leave the chip underfilled
+  const int waves_per_wg = (t.block + 63) / 64;
+  long waves = (long)t.gx * waves_per_wg;
+  int groups = 1;
+  if (c->user_bpg[dir] > 0) {
+    groups = (c->nb + c->user_bpg[dir] - 1) / c->user_bpg[dir];
+  } else {
+    const long target = 8192;
+    if (waves < target) groups = (int)std::min<long>(c->nb, (target + waves - 1) / std::max<long>(waves, 1));
+    if (groups < 1) groups = 1;
+  }
+  t.bpg = (c->nb + groups - 1) / groups;
+  t.groups = (c->nb + t.bpg - 1) / t.bpg;
+  return t;
+}
+
+int refresh_tiling(fall3d_ads_ctx *c) {
+  for (int d = 0; d < 3; ++d) {
+    Tiling t = choose_tiling(c, d);
+    if (t.S <= 0 || t.cpt < 2 || !sweep_cpt_supported(t.cpt))
+      return fail(c, FALL3D_ADS_EINVAL, "no supported tiling for axis %d (n=%d cpt=%d tx=%d)", d, axis_n(c, d),
+                  t.cpt, t.TX);
+    c->tl[d] = t;
+    const size_t need = (size_t)t.gx * t.groups;
+    if (c->part_cap[d + 1] < need) {
+      dfree(c->part[d + 1]);
+      HIPCHK(c, dalloc(&c->part[d + 1], need * 4));
+      c->part_cap[d + 1] = need;
+    }
+  }
+  return FALL3D_ADS_OK;
+}
+
+// ---- sweeps ---------------------------------------------------------------------
+struct SweepOut {
+  int nblocks = 0;
+};
+
+int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *vel1, const float *vel2,
+              const float *kd, const float *vsd, int write_ghosts, SweepOut *out) {
+  const Tiling &t = c->tl[dir];
+  SweepArgs a;
+  memset(&a, 0, sizeof(a));
+  a.chi = c->chi;
+  a.clo = c->comp ? c->clo : nullptr;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.n = axis_n(c, dir);
+  a.S = t.S;
+  a.TX = t.TX;
+  a.ntrans = t.ntrans;
+  a.bin0 = 0;
+  a.nbin = t.bpg;
+  a.nbins = c->nb;
+  a.tm = c->tm;
+  a.write_ghosts = write_ghosts;
+  a.dt = (float)dt;
+  a.stime = (float)stime;
+  a.vel1 = vel1;
+  a.vel2 = vel2;
+  a.kd = kd;
+  a.vs = vsd;
+  a.hdxp = c->hdxp[dir];
+  a.idxb = c->idxb[dir];
+  a.dxb64 = c->dxb64[dir];
+  if (dir == 0) {
+    a.areaA = c->dxb64[1];
+    a.areaB = c->dxb64[2];
+  } else if (dir == 1) {
+    a.areaA = c->dxb64[0];
+    a.areaB = c->dxb64[2];
+  } else {
+    a.areaA = c->dxb64[0];
+    a.areaB = c->dxb64[1];
+  }
+  a.acum = c->acum;
+  a.partials = c->part[dir + 1];
+  dim3 grid(t.gx, t.groups), block(t.block);
+  const size_t shmem = sizeof(float) * 8 * (size_t)t.block;
+  {
+    ProfScope ps(c, KID_SWEEP_X + dir);
+    hipError_t e = launch_sweep(dir, t.cpt, c->limiter, c->comp != 0, a, grid, block, shmem, c->stream);
+    if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "sweep %d launch: %s", dir, hipGetErrorString(e));
+  }
+  out->nblocks = t.gx * t.groups;
+  return FALL3D_ADS_OK;
+}
+
+void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks) {
+  // partial slots: 0 ref beg, 1 ref end, 2 ledger beg, 3 ledger end
+  const int dst[4] = {ACC_FLUX + 2 * dir, ACC_FLUX + 2 * dir + 1, ACC_LEDGER + 2 * dir, ACC_LEDGER + 2 * dir + 1};
+  for (int s = 0; s < 4; ++s) {
+    j.part[j.count] = c->part[dir + 1];
+    j.nblocks[j.count] = nblocks;
+    j.slot[j.count] = s;
+    j.dst[j.count] = dst[s];
+    j.count++;
+  }
+}
+
+int run_finalize(fall3d_ads_ctx *c, const ReduceJob &j) {
+  ProfScope ps(c, KID_REDUCE);
+  hipError_t e = finalize(c->acc, j, c->stream);
+  if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "finalize: %s", hipGetErrorString(e));
+  return FALL3D_ADS_OK;
+}
+
+// upload a host array of the context kind into device staging
+int upload(fall3d_ads_ctx *c, const void *h, size_t nelem, void **dptr) {
+  const size_t bytes = nelem * (size_t)c->kind;
+  int rc = ensure_stage(c, bytes);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->dstage, h, bytes, hipMemcpyHostToDevice, c->stream));
+  *dptr = c->dstage;
+  return FALL3D_ADS_OK;
+}
+
+int need_grid(fall3d_ads_ctx *c) {
+  if (!c->grid_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_grid has not been called");
+  return FALL3D_ADS_OK;
+}
+
+size_t n_u(const fall3d_ads_ctx *c) { return (size_t)(c->nx + 3) * c->ny * c->nz; }
+size_t n_v(const fall3d_ads_ctx *c) { return (size_t)(c->ny + 3) * c->nx * c->nz; }
+size_t n_w(const fall3d_ads_ctx *c) { return (size_t)(c->nz + 3) * c->nx * c->ny; }
+size_t n_k1(const fall3d_ads_ctx *c) { return (size_t)(c->nx + 4) * c->ny * c->nz; }
+size_t n_k2(const fall3d_ads_ctx *c) { return (size_t)(c->ny + 4) * c->nx * c->nz; }
+size_t n_k3(const fall3d_ads_ctx *c) { return (size_t)(c->nz + 4) * c->nx * c->ny; }
+size_t n_vs(const fall3d_ads_ctx *c) { return (size_t)c->nx * c->ny * (c->nz + 3) * c->nb; }
+
+// reference layout -> device layout for each coefficient field
+int up_u(fall3d_ads_ctx *c, const void *h, float *d) {
+  void *s;
+  int rc = upload(c, h, n_u(c), &s);
+  if (rc) return rc;
+  HIPCHK(c, to_f32(s, c->kind, d, n_u(c), c->stream));
+  return FALL3D_ADS_OK;
+}
+int up_v(fall3d_ads_ctx *c, const void *h, float *d) {
+  void *s;
+  int rc = upload(c, h, n_v(c), &s);
+  if (rc) return rc;
+  HIPCHK(c, transpose_to_f32(s, c->kind, d, c->ny + 3, c->nx, c->nz, c->stream));
+  return FALL3D_ADS_OK;
+}
+int up_w(fall3d_ads_ctx *c, const void *h, float *d) {
+  void *s;
+  int rc = upload(c, h, n_w(c), &s);
+  if (rc) return rc;
+  HIPCHK(c, transpose_to_f32(s, c->kind, d, c->nz + 3, c->nx * c->ny, 1, c->stream));
+  return FALL3D_ADS_OK;
+}
+int up_k1(fall3d_ads_ctx *c, const void *h) {
+  void *s;
+  int rc = upload(c, h, n_k1(c), &s);
+  if (rc) return rc;
+  HIPCHK(c, to_f32(s, c->kind, c->k1, n_k1(c), c->stream));
+  return FALL3D_ADS_OK;
+}
+int up_k2(fall3d_ads_ctx *c, const void *h) {
+  void *s;
+  int rc = upload(c, h, n_k2(c), &s);
+  if (rc) return rc;
+  HIPCHK(c, transpose_to_f32(s, c->kind, c->k2, c->ny + 4, c->nx, c->nz, c->stream));
+  return FALL3D_ADS_OK;
+}
+int up_k3(fall3d_ads_ctx *c, const void *h) {
+  void *s;
+  int rc = upload(c, h, n_k3(c), &s);
+  if (rc) return rc;
+  HIPCHK(c, transpose_to_f32(s, c->kind, c->k3, c->nz + 4, c->nx * c->ny, 1, c->stream));
+  return FALL3D_ADS_OK;
+}
+int up_vs(fall3d_ads_ctx *c, const void *h) {
+  void *s;
+  int rc = upload(c, h, n_vs(c), &s);
+  if (rc) return rc;
+  HIPCHK(c, to_f32(s, c->kind, c->vs, n_vs(c), c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int up_state(fall3d_ads_ctx *c, const void *h) {
+  void *s;
+  int rc = upload(c, h, c->ncell_c, &s);
+  if (rc) return rc;
+  HIPCHK(c, state_in(s, c->kind, c->chi, c->comp ? c->clo : nullptr, c->ncell_c, c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int down_state(fall3d_ads_ctx *c, void *h) {
+  const size_t bytes = c->ncell_c * (size_t)c->kind;
+  int rc = ensure_stage(c, bytes);
+  if (rc) return rc;
+  HIPCHK(c, state_out(c->chi, c->comp ? c->clo : nullptr, c->dstage, c->kind, c->ncell_c, c->stream));
+  HIPCHK(c, hipMemcpyAsync(h, c->dstage, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int up_acum(fall3d_ads_ctx *c, const void *h) {
+  void *s;
+  int rc = upload(c, h, c->nacum, &s);
+  if (rc) return rc;
+  HIPCHK(c, to_f64(s, c->kind, c->acum, c->nacum, c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int down_acum(fall3d_ads_ctx *c, void *h) {
+  if (c->kind == 8) {
+    HIPCHK(c, hipMemcpyAsync(h, c->acum, c->nacum * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FALL3D_ADS_OK;
+  }
+  std::vector<double> tmp(c->nacum);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), c->acum, c->nacum * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float *o = (float *)h;
+  for (size_t i = 0; i < c->nacum; ++i) o[i] = (float)tmp[i];
+  return FALL3D_ADS_OK;
+}
+
+double host_val(const void *p, int kind, size_t i) {
+  return kind == 8 ? ((const double *)p)[i] : (double)((const float *)p)[i];
+}
+
+int set_acc(fall3d_ads_ctx *c, int idx, const double *vals, int n) {
+  HIPCHK(c, hipMemcpyAsync(c->acc + idx, vals, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int get_acc(fall3d_ads_ctx *c, int idx, double *vals, int n) {
+  HIPCHK(c, hipMemcpyAsync(vals, c->acc + idx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+// one drop-in sweep: flux pair in/out through acc[ACC_SCRATCH..]
+int dropin_sweep(fall3d_ads_ctx *c, int dir, double *fbeg, double *fend, double dt, void *cst, const void *kd,
+                 const void *w1, const void *w2, double stime, const void *vsh, void *acum) {
+  int rc;
+  if ((rc = need_grid(c))) return rc;
+  if (!cst || !kd || !fbeg || !fend) return fail(c, FALL3D_ADS_EINVAL, "null argument");
+  if ((rc = up_state(c, cst))) return rc;
+  const float *vel1 = nullptr, *vel2 = nullptr, *kdd = nullptr;
+  if (dir == 0) {
+    if ((rc = up_u(c, c->hu, c->u1))) return rc;
+    if ((rc = up_k1(c, kd))) return rc;
+    vel1 = vel2 = c->u1;
+    kdd = c->k1;
+    stime = 0.0;
+  } else if (dir == 1) {
+    if ((rc = up_v(c, c->hv, c->v1))) return rc;
+    if ((rc = up_k2(c, kd))) return rc;
+    vel1 = vel2 = c->v1;
+    kdd = c->k2;
+    stime = 0.0;
+  } else {
+    if ((rc = up_w(c, w1, c->w1))) return rc;
+    if ((rc = up_w(c, w2, c->w2))) return rc;
+    if ((rc = up_k3(c, kd))) return rc;
+    if ((rc = up_vs(c, vsh))) return rc;
+    if ((rc = up_acum(c, acum))) return rc;
+    vel1 = c->w1;
+    vel2 = c->w2;
+    kdd = c->k3;
+  }
+  double pair[2] = {*fbeg, *fend};
+  if ((rc = set_acc(c, ACC_FLUX + 2 * dir, pair, 2))) return rc;
+  SweepOut so;
+  if ((rc = run_sweep(c, dir, dt, stime, vel1, vel2, kdd, c->vs, 1, &so))) return rc;
+  ReduceJob j;
+  memset(&j, 0, sizeof(j));
+  add_sweep_jobs(j, c, dir, so.nblocks);
+  if ((rc = run_finalize(c, j))) return rc;
+  if ((rc = down_state(c, cst))) return rc;
+  if (dir == 2 && (rc = down_acum(c, acum))) return rc;
+  if ((rc = get_acc(c, ACC_FLUX + 2 * dir, pair, 2))) return rc;
+  *fbeg = pair[0];
+  *fend = pair[1];
+  // the meteo buffers now hold per-call data
+  c->meteo_ok = false;
+  return FALL3D_ADS_OK;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbins, int limiter, int time_marching,
+                          int real_kind, int state_mode) {
+  if (!pctx) return FALL3D_ADS_EINVAL;
+  *pctx = nullptr;
+  fall3d_ads_ctx *c = new (std::nothrow) fall3d_ads_ctx();
+  if (!c) return FALL3D_ADS_EINVAL;
+  *pctx = c;
+  if (nx < 2 || ny < 2 || nz < 2 || nbins < 1)
+    return fail(c, FALL3D_ADS_EINVAL, "invalid grid %d x %d x %d x %d (each axis needs >= 2 cells)", nx, ny, nz,
+                nbins);
+  if (limiter < 1 || limiter > 3) return fail(c, FALL3D_ADS_EINVAL, "invalid limiter %d", limiter);
+  if (time_marching != 1 && time_marching != 2)
+    return fail(c, FALL3D_ADS_EINVAL, "invalid time_marching %d", time_marching);
+  if (real_kind != 4 && real_kind != 8) return fail(c, FALL3D_ADS_EINVAL, "invalid real kind %d", real_kind);
+  if (state_mode != 0 && state_mode != 1) return fail(c, FALL3D_ADS_EINVAL, "invalid state mode %d", state_mode);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(c, FALL3D_ADS_EHIP, "no HIP device available");
+  c->nx = nx;
+  c->ny = ny;
+  c->nz = nz;
+  c->nb = nbins;
+  c->limiter = limiter;
+  c->tm = time_marching;
+  c->kind = real_kind;
+  c->comp = state_mode;
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->own_stream = true;
+  c->ncell_c = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * nbins;
+  c->nacum = (size_t)(nx + 4) * (ny + 4) * nbins;
+  HIPCHK(c, dalloc(&c->chi, c->ncell_c));
+  HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
+  if (c->comp) {
+    HIPCHK(c, dalloc(&c->clo, c->ncell_c));
+    HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_c * sizeof(float), c->stream));
+  }
+  HIPCHK(c, dalloc(&c->acum, c->nacum));
+  HIPCHK(c, hipMemsetAsync(c->acum, 0, c->nacum * sizeof(double), c->stream));
+  HIPCHK(c, dalloc(&c->u1, n_u(c)));
+  HIPCHK(c, dalloc(&c->u2, n_u(c)));
+  HIPCHK(c, dalloc(&c->v1, n_v(c)));
+  HIPCHK(c, dalloc(&c->v2, n_v(c)));
+  HIPCHK(c, dalloc(&c->w1, n_w(c)));
+  HIPCHK(c, dalloc(&c->w2, n_w(c)));
+  HIPCHK(c, dalloc(&c->k1, n_k1(c)));
+  HIPCHK(c, dalloc(&c->k2, n_k2(c)));
+  HIPCHK(c, dalloc(&c->k3, n_k3(c)));
+  HIPCHK(c, dalloc(&c->vs, n_vs(c)));
+  HIPCHK(c, dalloc(&c->acc, ACC_LEN));
+  HIPCHK(c, hipMemsetAsync(c->acc, 0, ACC_LEN * sizeof(double), c->stream));
+  c->red_cap = 4096;
+  HIPCHK(c, dalloc(&c->red_part, c->red_cap));
+  for (int d = 0; d < 3; ++d) {
+    const int n = axis_n(c, d);
+    HIPCHK(c, dalloc(&c->dxp64[d], n + 4));
+    HIPCHK(c, dalloc(&c->dxb64[d], n + 3));
+    HIPCHK(c, dalloc(&c->hdxp[d], n + 3));
+    HIPCHK(c, dalloc(&c->idxb[d], n + 3));
+  }
+  const size_t ksz = (size_t)real_kind;
+  c->hu = calloc(n_u(c), ksz);
+  c->hv = calloc(n_v(c), ksz);
+  c->hw = calloc(n_w(c), ksz);
+  if (!c->hu || !c->hv || !c->hw) return fail(c, FALL3D_ADS_EINVAL, "host allocation failed");
+  int rc = refresh_tiling(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_release(fall3d_ads_ctx *c) {
+  if (!c) return FALL3D_ADS_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->evpool) (void)hipEventDestroy(e);
+  void *ptrs[] = {c->chi, c->clo, c->acum, c->u1, c->u2, c->v1, c->v2, c->w1, c->w2, c->k1, c->k2, c->k3, c->vs,
+                  c->src_idx, c->src_rate, c->src_meas, c->src_part, c->acc, c->part[1], c->part[2],
+                  c->part[3], c->red_part, c->dstage};  // part[0] aliases src_part
+  for (void *p : ptrs) dfree(p);
+  for (int d = 0; d < 3; ++d) {
+    dfree(c->dxp64[d]);
+    dfree(c->dxb64[d]);
+    dfree(c->hdxp[d]);
+    dfree(c->idxb[d]);
+  }
+  free(c->hu);
+  free(c->hv);
+  free(c->hw);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FALL3D_ADS_OK;
+}
+
+const char *fall3d_ads_last_error(const fall3d_ads_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int fall3d_ads_set_grid(fall3d_ads_ctx *c, const void *dX1_p, const void *dX1_b, const void *dX2_p,
+                        const void *dX2_b, const void *dX3_p, const void *dX3_b) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  const void *P[3] = {dX1_p, dX2_p, dX3_p};
+  const void *B[3] = {dX1_b, dX2_b, dX3_b};
+  for (int d = 0; d < 3; ++d) {
+    if (!P[d] || !B[d]) return fail(c, FALL3D_ADS_EINVAL, "null grid array");
+    const int n = axis_n(c, d);
+    std::vector<double> p(n + 4), b(n + 3);
+    for (int i = 0; i < n + 4; ++i) p[i] = host_val(P[d], c->kind, i);
+    for (int i = 0; i < n + 3; ++i) b[i] = host_val(B[d], c->kind, i);
+    for (int i = 0; i < n + 4; ++i)
+      if (!(p[i] > 0.0)) return fail(c, FALL3D_ADS_EINVAL, "non-positive dX%d_p at index %d", d + 1, i);
+    for (int i = 0; i < n + 3; ++i)
+      if (!(b[i] > 0.0)) return fail(c, FALL3D_ADS_EINVAL, "non-positive dX%d_b at index %d", d + 1, i);
+    // 0.5/dxp(f-1) for faces f = 0..n+2 (dxp position f-1 sits at index f)
+    std::vector<float> h(n + 3), ib(n + 3);
+    for (int f = 0; f < n + 3; ++f) h[f] = (float)(0.5 / p[f]);
+    for (int i = 0; i < n + 3; ++i) ib[i] = (float)(1.0 / b[i]);
+    HIPCHK(c, hipMemcpy(c->dxp64[d], p.data(), sizeof(double) * (n + 4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->dxb64[d], b.data(), sizeof(double) * (n + 3), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->hdxp[d], h.data(), sizeof(float) * (n + 3), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->idxb[d], ib.data(), sizeof(float) * (n + 3), hipMemcpyHostToDevice));
+    c->hdxp64[d] = p;
+    c->hdxb64[d] = b;
+  }
+  c->grid_ok = true;
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_get_vcomps(fall3d_ads_ctx *c, void **u, void **v, void **w) {
+  if (!c || !u || !v || !w) return FALL3D_ADS_EINVAL;
+  *u = c->hu;
+  *v = c->hv;
+  *w = c->hw;
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_update_horizontal_vcomps(fall3d_ads_ctx *c, const void *u1, const void *u2, const void *v1,
+                                        const void *v2, double stime) {
+  if (!c || !u1 || !u2 || !v1 || !v2) return FALL3D_ADS_EINVAL;
+  // interpolate on the GPU in the reference layouts (FP32 results), then
+  // refresh the host CB%u / CB%v arrays the caller can read
+  int rc;
+  const size_t nu = n_u(c), nv = n_v(c);
+  float *a = nullptr, *b = nullptr;
+  HIPCHK(c, dalloc(&a, std::max(nu, nv)));
+  HIPCHK(c, dalloc(&b, std::max(nu, nv)));
+  const void *src[2][2] = {{u1, u2}, {v1, v2}};
+  void *dst[2] = {c->hu, c->hv};
+  const size_t cnt[2] = {nu, nv};
+  for (int q = 0; q < 2; ++q) {
+    void *s;
+    if ((rc = upload(c, src[q][0], cnt[q], &s))) { dfree(a); dfree(b); return rc; }
+    HIPCHK(c, to_f32(s, c->kind, a, cnt[q], c->stream));
+    if ((rc = upload(c, src[q][1], cnt[q], &s))) { dfree(a); dfree(b); return rc; }
+    HIPCHK(c, to_f32(s, c->kind, b, cnt[q], c->stream));
+    HIPCHK(c, lerp(a, b, a, (float)stime, cnt[q], c->stream));
+    if ((rc = ensure_stage(c, cnt[q] * 8))) { dfree(a); dfree(b); return rc; }
+    HIPCHK(c, state_out(a, nullptr, c->dstage, c->kind, cnt[q], c->stream));
+    HIPCHK(c, hipMemcpyAsync(dst[q], c->dstage, cnt[q] * c->kind, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(a);
+  dfree(b);
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_solve_along_x(fall3d_ads_ctx *c, double *W_flux, double *E_flux, double dt, void *cst,
+                             const void *k1) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  return dropin_sweep(c, 0, W_flux, E_flux, dt, cst, k1, nullptr, nullptr, 0.0, nullptr, nullptr);
+}
+
+int fall3d_ads_solve_along_y(fall3d_ads_ctx *c, double *S_flux, double *N_flux, double dt, void *cst,
+                             const void *k2) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  return dropin_sweep(c, 1, S_flux, N_flux, dt, cst, k2, nullptr, nullptr, 0.0, nullptr, nullptr);
+}
+
+int fall3d_ads_solve_along_z(fall3d_ads_ctx *c, double *D_flux, double *U_flux, double dt, void *cst, void *acum,
+                             const void *k3, const void *w1, const void *w2, double stime, const void *vs) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  if (!acum || !w1 || !w2 || !vs) return fail(c, FALL3D_ADS_EINVAL, "null argument");
+  return dropin_sweep(c, 2, D_flux, U_flux, dt, cst, k3, w1, w2, stime, vs, acum);
+}
+
+int fall3d_ads_solve_1d(fall3d_ads_ctx *c, void *cl, const void *dxp, const void *dxb, const void *u,
+                        const void *k, double dt, const double bvalue[2], double flux[2], const int bcode[2], int ips,
+                        int ipe, int limiter) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  if (!cl || !dxp || !dxb || !u || !k || !bvalue || !flux || !bcode) return fail(c, FALL3D_ADS_EINVAL, "null argument");
+  const int n = ipe - ips + 1;
+  if (n < 2 || n > 1500) return fail(c, FALL3D_ADS_EINVAL, "solve_1d: line length %d out of range [2,1500]", n);
+  if (limiter < 1 || limiter > 3) return fail(c, FALL3D_ADS_EINVAL, "invalid limiter %d", limiter);
+  for (int s = 0; s < 2; ++s)
+    if (bcode[s] < -1 || bcode[s] > 2) return fail(c, FALL3D_ADS_EINVAL, "invalid bcode %d", bcode[s]);
+  std::vector<double> hc(n + 4), hp(n + 4), hb(n + 3), hu(n + 3), hk(n + 4);
+  for (int i = 0; i < n + 4; ++i) {
+    hc[i] = host_val(cl, c->kind, i);
+    hp[i] = host_val(dxp, c->kind, i);
+    hk[i] = host_val(k, c->kind, i);
+  }
+  for (int i = 0; i < n + 3; ++i) {
+    hb[i] = host_val(dxb, c->kind, i);
+    hu[i] = host_val(u, c->kind, i);
+  }
+  double *d = nullptr;
+  const size_t tot = 4 * (size_t)(n + 4) + (n + 3) * 2 + 2;
+  HIPCHK(c, dalloc(&d, tot));
+  double *dc = d, *dp = dc + (n + 4), *dk = dp + (n + 4), *db = dk + (n + 4), *du = db + (n + 3), *df = du + (n + 3);
+  HIPCHK(c, hipMemcpyAsync(dc, hc.data(), 8 * (n + 4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dp, hp.data(), 8 * (n + 4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dk, hk.data(), 8 * (n + 4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(db, hb.data(), 8 * (n + 3), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(du, hu.data(), 8 * (n + 3), hipMemcpyHostToDevice, c->stream));
+  Solve1DArgs a;
+  memset(&a, 0, sizeof(a));
+  a.c = dc;
+  a.dxp = dp;
+  a.dxb = db;
+  a.u = du;
+  a.k = dk;
+  a.flux = df;
+  a.dt = dt;
+  a.bvalue[0] = bvalue[0];
+  a.bvalue[1] = bvalue[1];
+  a.bcode[0] = bcode[0];
+  a.bcode[1] = bcode[1];
+  a.n = n;
+  a.limiter = limiter;
+  hipError_t e = solve_1d(a, c->stream);
+  if (e != hipSuccess) {
+    dfree(d);
+    return fail(c, FALL3D_ADS_EHIP, "solve_1d: %s", hipGetErrorString(e));
+  }
+  double fl[2];
+  HIPCHK(c, hipMemcpyAsync(hc.data(), dc, 8 * (n + 4), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(fl, df, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  dfree(d);
+  for (int i = 0; i < n + 4; ++i) {
+    if (c->kind == 8)
+      ((double *)cl)[i] = hc[i];
+    else
+      ((float *)cl)[i] = (float)hc[i];
+  }
+  flux[0] = fl[0];
+  flux[1] = fl[1];
+  return FALL3D_ADS_OK;
+}
+
+// ---- resident group ---------------------------------------------------------------
+
+int fall3d_ads_set_meteo(fall3d_ads_ctx *c, const void *u1, const void *u2, const void *v1, const void *v2,
+                         const void *w1, const void *w2, const void *k1, const void *k2, const void *k3,
+                         const void *vs) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  if (!u1 || !v1 || !w1 || !k1 || !k2 || !k3 || !vs) return fail(c, FALL3D_ADS_EINVAL, "null meteo array");
+  int rc;
+  if ((rc = up_u(c, u1, c->u1))) return rc;
+  if ((rc = up_u(c, u2 ? u2 : u1, c->u2))) return rc;
+  if ((rc = up_v(c, v1, c->v1))) return rc;
+  if ((rc = up_v(c, v2 ? v2 : v1, c->v2))) return rc;
+  if ((rc = up_w(c, w1, c->w1))) return rc;
+  if ((rc = up_w(c, w2 ? w2 : w1, c->w2))) return rc;
+  if ((rc = up_k1(c, k1))) return rc;
+  if ((rc = up_k2(c, k2))) return rc;
+  if ((rc = up_k3(c, k3))) return rc;
+  if ((rc = up_vs(c, vs))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->meteo_ok = true;
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  int rc;
+  if ((rc = need_grid(c))) return rc;
+  dfree(c->src_idx);
+  dfree(c->src_rate);
+  dfree(c->src_meas);
+  dfree(c->src_part);
+  c->src_idx = nullptr;
+  c->src_rate = c->src_meas = c->src_part = nullptr;
+  c->part[0] = nullptr;
+  c->nsrc = 0;
+  if (!s) return FALL3D_ADS_OK;
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  std::vector<size_t> idx;
+  std::vector<double> rate, meas;
+  // Sources/mod_F3D.F90:1456-1473: c += dt*s/vol, vol = dX1_p dX2_p dX3_p (Hm = 1)
+  for (int b = 0; b < c->nb; ++b)
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          const size_t si = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)(k - 1) + (size_t)nz * b));
+          const double v = host_val(s, c->kind, si);
+          if (v == 0.0) continue;
+          const double vol = c->hdxp64[0][i + 1] * c->hdxp64[1][j + 1] * c->hdxp64[2][k + 1];
+          idx.push_back((size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b)));
+          rate.push_back(v / vol);
+          meas.push_back(c->hdxb64[0][i] * c->hdxb64[1][j] * c->hdxb64[2][k]);
+        }
+  c->nsrc = (int)idx.size();
+  if (c->nsrc == 0) return FALL3D_ADS_OK;
+  HIPCHK(c, dalloc(&c->src_idx, idx.size()));
+  HIPCHK(c, dalloc(&c->src_rate, idx.size()));
+  HIPCHK(c, dalloc(&c->src_meas, idx.size()));
+  const size_t nblk = (idx.size() + 255) / 256;
+  HIPCHK(c, dalloc(&c->src_part, nblk * 4));
+  c->part[0] = c->src_part;
+  HIPCHK(c, hipMemcpy(c->src_idx, idx.data(), sizeof(size_t) * idx.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->src_rate, rate.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->src_meas, meas.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_set_state(fall3d_ads_ctx *c, const void *cst, const void *acum) {
+  if (!c || !cst) return FALL3D_ADS_EINVAL;
+  int rc;
+  if ((rc = up_state(c, cst))) return rc;
+  if (acum) {
+    if ((rc = up_acum(c, acum))) return rc;
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->acum, 0, c->nacum * sizeof(double), c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_get_state(fall3d_ads_ctx *c, void *cst, void *acum) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  int rc;
+  if (cst && (rc = down_state(c, cst))) return rc;
+  if (acum && (rc = down_acum(c, acum))) return rc;
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_get_time_step(fall3d_ads_ctx *c, double stime, int criterion, double cfl, double *dt) {
+  if (!c || !dt) return FALL3D_ADS_EINVAL;
+  int rc;
+  if ((rc = need_grid(c))) return rc;
+  if (!c->meteo_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_meteo has not been called");
+  if (criterion != 1 && criterion != 2) return fail(c, FALL3D_ADS_EINVAL, "invalid CFL criterion %d", criterion);
+  CflArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.nb = c->nb;
+  a.criterion = criterion;
+  a.stime = stime;
+  a.cfl = cfl;
+  a.u1 = c->u1; a.u2 = c->u2; a.v1 = c->v1; a.v2 = c->v2; a.w1 = c->w1; a.w2 = c->w2;
+  a.k1 = c->k1; a.k2 = c->k2; a.k3 = c->k3; a.vs = c->vs;
+  a.dx1p = c->dxp64[0];
+  a.dx2p = c->dxp64[1];
+  a.dx3p = c->dxp64[2];
+  HIPCHK(c, cfl_time_step(a, c->red_part, c->red_cap - 1, c->red_part + (c->red_cap - 1), c->stream));
+  HIPCHK(c, hipMemcpyAsync(dt, c->red_part + (c->red_cap - 1), sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_step(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  int rc;
+  if ((rc = need_grid(c))) return rc;
+  if (!c->meteo_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_meteo has not been called");
+  ReduceJob j;
+  memset(&j, 0, sizeof(j));
+  if (c->nsrc > 0) {
+    int nblk = 0;
+    {
+      ProfScope ps(c, KID_SOURCE);
+      hipError_t e = source_inject(c->chi, c->comp ? c->clo : nullptr, c->src_idx, c->src_rate, c->src_meas,
+                                   c->nsrc, dt, c->src_part, &nblk, c->stream);
+      if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "source: %s", hipGetErrorString(e));
+    }
+    j.part[j.count] = c->src_part;
+    j.nblocks[j.count] = nblk;
+    j.slot[j.count] = 0;
+    j.dst[j.count] = ACC_INJECTED;
+    j.count++;
+  }
+  SweepOut so;
+  const int order[3] = {(iiter % 2 == 0) ? 0 : 1, (iiter % 2 == 0) ? 1 : 0, 2};
+  for (int q = 0; q < 3; ++q) {
+    const int d = order[q];
+    const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
+    const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
+    const float *kd = d == 0 ? c->k1 : (d == 1 ? c->k2 : c->k3);
+    if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, c->vs, 1, &so))) return rc;
+    add_sweep_jobs(j, c, d, so.nblocks);
+  }
+  return run_finalize(c, j);
+}
+
+int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = fall3d_ads_step(c, dt, stime, iiter0 + s);
+    if (rc) return rc;
+  }
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_get_fluxes(fall3d_ads_ctx *c, double flux[6]) {
+  if (!c || !flux) return FALL3D_ADS_EINVAL;
+  return get_acc(c, ACC_FLUX, flux, 6);
+}
+
+int fall3d_ads_set_fluxes(fall3d_ads_ctx *c, const double flux[6]) {
+  if (!c || !flux) return FALL3D_ADS_EINVAL;
+  int rc = set_acc(c, ACC_FLUX, flux, 6);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_get_budget(fall3d_ads_ctx *c, double out[FALL3D_ADS_BUDGET_LEN]) {
+  if (!c || !out) return FALL3D_ADS_EINVAL;
+  int rc;
+  if ((rc = need_grid(c))) return rc;
+  double *o = c->red_part + (c->red_cap - 2);
+  // mass with mass-point spacings (dxp position p at index p+1) and with cell sizes
+  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->nx, c->ny, c->nz, c->nb, c->dxp64[0], c->dxp64[1],
+                 c->dxp64[2], 1, c->red_part, c->red_cap - 2, o, c->stream));
+  double mv = 0.0, mc = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&mv, o, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->nx, c->ny, c->nz, c->nb, c->dxb64[0], c->dxb64[1],
+                 c->dxb64[2], 0, c->red_part, c->red_cap - 2, o, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&mc, o, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  double acc[ACC_LEN];
+  HIPCHK(c, hipMemcpyAsync(acc, c->acc, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[FALL3D_ADS_BUDGET_MASS_VOLUME] = mv;
+  out[FALL3D_ADS_BUDGET_MASS_CELLS] = mc;
+  out[FALL3D_ADS_BUDGET_GROUND] = fabs(acc[ACC_FLUX + 4]);
+  out[FALL3D_ADS_BUDGET_LATERAL] =
+      fabs(acc[ACC_FLUX + 0]) + fabs(acc[ACC_FLUX + 1]) + fabs(acc[ACC_FLUX + 2]) + fabs(acc[ACC_FLUX + 3]) +
+      fabs(acc[ACC_FLUX + 5]);
+  out[FALL3D_ADS_BUDGET_INJECTED] = acc[ACC_INJECTED];
+  for (int s = 0; s < 6; ++s) out[FALL3D_ADS_BUDGET_LEDGER_W + s] = acc[ACC_LEDGER + s];
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_set_stream(fall3d_ads_ctx *c, void *stream) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (stream) {
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+  } else if (!c->own_stream) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = true;
+  }
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_synchronize(fall3d_ads_ctx *c) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_set_tiling(fall3d_ads_ctx *c, int dir, int cells_per_thread, int columns, int bins_per_group) {
+  if (!c || dir < 0 || dir > 2) return FALL3D_ADS_EINVAL;
+  if (cells_per_thread && !sweep_cpt_supported(cells_per_thread))
+    return fail(c, FALL3D_ADS_EINVAL, "unsupported cells per thread %d", cells_per_thread);
+  if (columns && (columns < 1 || columns > 64)) return fail(c, FALL3D_ADS_EINVAL, "columns %d out of range", columns);
+  const int oc = c->user_cpt[dir], ot = c->user_tx[dir], ob = c->user_bpg[dir];
+  c->user_cpt[dir] = cells_per_thread;
+  c->user_tx[dir] = columns;
+  c->user_bpg[dir] = bins_per_group;
+  int rc = refresh_tiling(c);
+  if (rc) {
+    c->user_cpt[dir] = oc;
+    c->user_tx[dir] = ot;
+    c->user_bpg[dir] = ob;
+    (void)refresh_tiling(c);
+  }
+  return rc;
+}
+
+int fall3d_ads_set_profiling(fall3d_ads_ctx *c, int on) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  drain_events(c);
+  c->prof = on != 0;
+  for (int k = 0; k < KID_COUNT; ++k) {
+    c->kt_ms[k] = 0.0;
+    c->kt_n[k] = 0;
+  }
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_get_kernel_time(fall3d_ads_ctx *c, int kernel, double *total_ms, long long *launches) {
+  if (!c || kernel < 0 || kernel >= KID_COUNT || !total_ms || !launches) return FALL3D_ADS_EINVAL;
+  drain_events(c);
+  *total_ms = c->kt_ms[kernel];
+  *launches = c->kt_n[kernel];
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_sweep_bytes(fall3d_ads_ctx *c, int dir, double *bytes) {
+  if (!c || !bytes || dir < 0 || dir > 2) return FALL3D_ADS_EINVAL;
+  const double cells = (double)c->nx * c->ny * c->nz;
+  const double st = c->comp ? 16.0 : 8.0;  // state read + write per cell and bin
+  // coefficient fields read once per sweep: two wind snapshots + diffusion
+  double b = cells * (c->nb * st + 3.0 * 4.0);
+  if (dir == 2) b += cells * c->nb * 4.0;  // settling velocity per bin
+  *bytes = b;
+  return FALL3D_ADS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// ads_internal.h -- shared declarations of the gfx950 ADS kernels and the
+// C-ABI host code (libfall3d_ads_hip.so). Not a public header.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace f3d {
+
+// Device copies of the wind/diffusion/settling fields, FP32, every array
+// x-fastest so that y and z sweeps read them coalesced across x:
+//   u1,u2 (0:nx+2, ny, nz)       idx = f + (nx+3)*((j-1) + ny*(k-1))
+//   v1,v2 (nx, 0:ny+2, nz)       idx = (i-1) + nx*(f + (ny+3)*(k-1))
+//   w1,w2 (nx, ny, 0:nz+2)       idx = (i-1) + nx*((j-1) + ny*f)
+//   k1    (-1:nx+2, ny, nz)      idx = (i+1) + (nx+4)*((j-1) + ny*(k-1))
+//   k2    (nx, -1:ny+2, nz)      idx = (i-1) + nx*((j+1) + (ny+4)*(k-1))
+//   k3    (nx, ny, -1:nz+2)      idx = (i-1) + nx*((j-1) + ny*(k+1))
+//   vs    (nx, ny, 0:nz+2, nb)   idx = (i-1) + nx*((j-1) + ny*(f + (nz+3)*b))
+// The concentration keeps the reference layout c(-1:nx+2,-1:ny+2,-1:nz+2,nb).
+
+struct SweepArgs {
+  float *chi;             // state (hi part)
+  float *clo;             // low part (compensated state) or nullptr
+  int nx, ny, nz;         // interior extents
+  int n;                  // cells along the sweep axis
+  int S;                  // segments (threads) per line
+  int TX;                 // lines per workgroup (x-columns for y/z sweeps)
+  int ntrans;             // lines along the first transverse axis handled by the grid
+  int bin0, nbin;         // bins handled per workgroup (blockIdx.y selects the group)
+  int nbins;              // total bins
+  int tm;                 // 1 Euler, 2 RK4
+  int write_ghosts;       // store the free-flow ghosts back into c
+  float dt, stime;
+  const float *vel1, *vel2;  // face winds along the sweep axis (device layout)
+  const float *kd;           // diffusion at mass points along the sweep axis
+  const float *vs;           // settling velocity (z sweep only)
+  const float *hdxp;         // 0.5/dxp(f-1) for faces f = 0..n+2
+  const float *idxb;         // 1/dxb(p) for cells p = 0..n+2
+  const double *dxb64;       // dxb(p) in FP64, p = 0..n+2 (mass measure of the ledger)
+  const double *areaA;       // FP64 transverse cell sizes for the face area (index 0..)
+  const double *areaB;
+  double *acum;              // ground accumulation (z sweep only), reference layout, FP64
+  double *partials;          // [gridDim.x*gridDim.y][4]: ref flux beg/end, ledger beg/end
+};
+
+// Sweep launchers (ads_sweep_*.hip). cpt = cells per thread.
+hipError_t launch_sweep(int dir, int cpt, int limiter, bool comp, const SweepArgs &a, dim3 grid, dim3 block,
+                        size_t shmem, hipStream_t st);
+bool sweep_cpt_supported(int cpt);
+extern const int kSweepCpt[];
+extern const int kSweepCptCount;
+
+}  // namespace f3d

@@ -1,0 +1,301 @@
+"""Synthetic ADS problems (grid, winds, diffusion, settling, sources).
+
+Every array is float64 in the layout the reference passes to its ADS
+routines (Fortran column-major, explicit halo bounds):
+
+  c    (-1:nx+2, -1:ny+2, -1:nz+2, nb)  Sources/mod_KindType.F90:389
+  s    ( 1:nx,    1:ny,    1:nz,   nb)  Sources/mod_KindType.F90:390
+  vs   ( 1:nx,    1:ny,    0:nz+2, nb)  Sources/mod_KindType.F90:391
+  acum (-1:nx+2, -1:ny+2, nb)           Sources/mod_KindType.F90:392
+  u    ( 0:nx+2,  1:ny,    1:nz)        Sources/mod_ADS.F90:110 (x fastest)
+  v    ( 0:ny+2,  1:nx,    1:nz)        Sources/mod_ADS.F90:111 (y fastest)
+  w    ( 0:nz+2,  1:nx,    1:ny)        Sources/mod_ADS.F90:112 (z fastest)
+  k1   (-1:nx+2,  1:ny,    1:nz)        Sources/mod_ADS.F90:249
+  k2   (-1:ny+2,  1:nx,    1:nz)        Sources/mod_ADS.F90:438
+  k3   (-1:nz+2,  1:nx,    1:ny)        Sources/mod_ADS.F90:634
+  dX1_p(-1:nx+2) dX1_b(0:nx+2) ...      Sources/mod_KindType.F90:486-491
+
+Face index f of a sweep axis is the face on the low side of cell f, so the
+boundary faces of an n-cell axis are f = 1 and f = n + 1; f = 0 and n + 2
+are halo faces that the solver never reads.
+
+The generators build the workloads the benchmark and the tests use:
+  puff_problem       -- 64x64x32, one bin, Gaussian puff, analytic shear wind
+                        with zero-flux lateral/top faces and an absorbing ground
+  operational_problem -- seeded Fourier-mode wind + shear, log-spaced settling
+                        velocities, vertical line source(s)
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Problem:
+    nx: int
+    ny: int
+    nz: int
+    nb: int
+    grid: dict
+    u1: np.ndarray
+    u2: np.ndarray
+    v1: np.ndarray
+    v2: np.ndarray
+    w1: np.ndarray
+    w2: np.ndarray
+    k1: np.ndarray
+    k2: np.ndarray
+    k3: np.ndarray
+    vs: np.ndarray
+    c0: np.ndarray
+    s: Optional[np.ndarray] = None
+    name: str = ""
+    meta: dict = dataclasses.field(default_factory=dict)
+
+    def new_c(self) -> np.ndarray:
+        return np.array(self.c0, order="F", copy=True)
+
+    def new_acum(self) -> np.ndarray:
+        return np.zeros((self.nx + 4, self.ny + 4, self.nb), order="F")
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def source_cells(self):
+        """Nonzero source entries as (i, j, k, b, rate) with 1-based i, j, k and 0-based b."""
+        if self.s is None:
+            return []
+        idx = np.argwhere(self.s != 0.0)
+        return [(int(i) + 1, int(j) + 1, int(k) + 1, int(b), float(self.s[i, j, k, b])) for i, j, k, b in idx]
+
+
+def _F(shape):
+    return np.zeros(shape, dtype=np.float64, order="F")
+
+
+def uniform_grid(nx, ny, nz, dx, dy, dz):
+    """Cartesian metrics of Sources/mod_Grid.F90:485-562 for uniform spacing."""
+    return {
+        "dX1_p": np.full(nx + 4, float(dx)), "dX1_b": np.full(nx + 3, float(dx)),
+        "dX2_p": np.full(ny + 4, float(dy)), "dX2_b": np.full(ny + 3, float(dy)),
+        "dX3_p": np.full(nz + 4, float(dz)), "dX3_b": np.full(nz + 3, float(dz)),
+    }
+
+
+def sigma_grid(nx, ny, nz, dx, dy, sigma, x3max):
+    """Vertical metrics from sigma levels gl_sigma(1:nz+1), Sources/mod_Grid.F90:541-562."""
+    sigma = np.asarray(sigma, dtype=np.float64)
+    assert sigma.shape == (nz + 1,)
+    g = uniform_grid(nx, ny, nz, dx, dy, 1.0)
+    db = np.zeros(nz + 3)  # index 0..nz+2
+    for k in range(1, nz + 1):
+        db[k] = (sigma[k] - sigma[k - 1]) * x3max
+    db[0] = db[1]
+    db[nz + 1] = db[nz]
+    db[nz + 2] = db[nz]
+    dp = np.zeros(nz + 4)  # index -1..nz+2 -> dp[k+1]
+    for k in range(1, nz):
+        zp = 0.5 * (sigma[k] + sigma[k - 1]) * x3max
+        zp1 = 0.5 * (sigma[k + 1] + sigma[k]) * x3max
+        dp[k + 1] = zp1 - zp
+    dp[0] = dp[2]
+    dp[1] = dp[2]
+    dp[nz + 1] = dp[nz]
+    dp[nz + 2] = dp[nz]
+    dp[nz + 3] = dp[nz]
+    g["dX3_b"] = db
+    g["dX3_p"] = dp
+    return g
+
+
+def constant_diffusion(nx, ny, nz, kh, kv):
+    """Constant K fills every element, halos included (Sources/mod_Phys.F90:479-480,590)."""
+    k1 = _F((nx + 4, ny, nz)); k1[...] = kh
+    k2 = _F((ny + 4, nx, nz)); k2[...] = kh
+    k3 = _F((nz + 4, nx, ny)); k3[...] = kv
+    return k1, k2, k3
+
+
+def puff_problem(nx=64, ny=64, nz=32, dx=5000.0, dy=5000.0, dz=500.0, nb=1, vset=0.5,
+                 kh=100.0, kv=10.0, sigma_cells=4.0, peak=1.0, zc=8000.0) -> Problem:
+    """Gaussian puff in a sheared wind with zero-flux laterals/top and an absorbing ground.
+
+    u = 10 + 5 sin(2 pi z_k / H) (H = domain top), v = 3, w = 0; boundary-face
+    normal velocities are 0 (zero flux, Sources/mod_ADS.F90:1243-1321); v_s on
+    every w-face except the top face (zero-flux top), so the ground face has
+    w_eff = -v_s (deposition).
+    """
+    H = nz * dz
+    g = uniform_grid(nx, ny, nz, dx, dy, dz)
+    zk = (np.arange(1, nz + 1) - 0.5) * dz
+    u = _F((nx + 3, ny, nz))
+    u[...] = (10.0 + 5.0 * np.sin(2.0 * math.pi * zk / H))[None, None, :]
+    u[0, :, :] = 0.0; u[1, :, :] = 0.0; u[nx + 1, :, :] = 0.0; u[nx + 2, :, :] = 0.0
+    v = _F((ny + 3, nx, nz)); v[...] = 3.0
+    v[0, :, :] = 0.0; v[1, :, :] = 0.0; v[ny + 1, :, :] = 0.0; v[ny + 2, :, :] = 0.0
+    w = _F((nz + 3, nx, ny))
+    k1, k2, k3 = constant_diffusion(nx, ny, nz, kh, kv)
+    vs = _F((nx, ny, nz + 3, nb))
+    vset = np.broadcast_to(np.asarray(vset, dtype=np.float64), (nb,))
+    for b in range(nb):
+        vs[:, :, 0:nz + 1, b] = vset[b]
+    c = _F((nx + 4, ny + 4, nz + 4, nb))
+    xc, yc = 0.5 * nx * dx, 0.5 * ny * dy
+    xi = (np.arange(1, nx + 1) - 0.5) * dx
+    yj = (np.arange(1, ny + 1) - 0.5) * dy
+    sx, sy, sz = sigma_cells * dx, sigma_cells * dy, sigma_cells * dz
+    gx = np.exp(-0.5 * ((xi - xc) / sx) ** 2)
+    gy = np.exp(-0.5 * ((yj - yc) / sy) ** 2)
+    gz = np.exp(-0.5 * ((zk - zc) / sz) ** 2)
+    for b in range(nb):
+        c[2:nx + 2, 2:ny + 2, 2:nz + 2, b] = peak * gx[:, None, None] * gy[None, :, None] * gz[None, None, :]
+    return Problem(nx, ny, nz, nb, g, u, u.copy(order="F"), v, v.copy(order="F"), w, w.copy(order="F"),
+                   k1, k2, k3, vs, c, None, name=f"puff{nx}x{ny}x{nz}x{nb}",
+                   meta={"dx": dx, "dy": dy, "dz": dz, "kh": kh, "kv": kv})
+
+
+class FourierWind:
+    """Divergence-free horizontal wind from a streamfunction of random modes.
+
+    psi(x, y) = sum_m a_m sin(kx_m x + ky_m y + phi_m); u = dpsi/dy, v = -dpsi/dx,
+    wavelengths log-uniform in [lmin, lmax], random directions and phases
+    (numpy PCG64 seeded). The field is scaled so that its maximum horizontal
+    speed over the domain equals `amplitude`; a linear shear `shear` (m/s per
+    km of height) is added to u.
+    """
+
+    def __init__(self, seed, nmodes=16, amplitude=15.0, lmin=50e3, lmax=1500e3, shear=2.0,
+                 amp_factors=None):
+        rng = np.random.default_rng(seed)
+        lam = np.exp(rng.uniform(math.log(lmin), math.log(lmax), nmodes))
+        theta = rng.uniform(0.0, 2.0 * math.pi, nmodes)
+        self.phi = rng.uniform(0.0, 2.0 * math.pi, nmodes)
+        kmag = 2.0 * math.pi / lam
+        self.kx = kmag * np.cos(theta)
+        self.ky = kmag * np.sin(theta)
+        # equal velocity amplitude per mode: a_m * |k_m| = 1
+        self.a = 1.0 / kmag
+        if amp_factors is not None:
+            self.a = self.a * np.asarray(amp_factors)
+        self.amplitude = amplitude
+        self.shear = shear
+        self.scale = 1.0
+
+    def _raw(self, x, y):
+        ph = np.multiply.outer(x, self.kx) + np.multiply.outer(y, self.ky) + self.phi
+        cs = np.cos(ph)
+        u = (cs * (self.a * self.ky)).sum(-1)
+        v = -(cs * (self.a * self.kx)).sum(-1)
+        return u, v
+
+    def calibrate(self, Lx, Ly, n=96):
+        xs = np.linspace(0.0, Lx, n)
+        ys = np.linspace(0.0, Ly, n)
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        u, v = self._raw(X, Y)
+        sp = np.sqrt(u * u + v * v).max()
+        self.scale = self.amplitude / sp if sp > 0 else 1.0
+
+    def uv(self, x, y):
+        u, v = self._raw(x, y)
+        return self.scale * u, self.scale * v
+
+
+def operational_problem(nx=360, ny=240, nz=60, nb=10, dx=5000.0, dy=5000.0, dz=300.0, seed=1,
+                        vmin=0.01, vmax=3.0, kh=100.0, kv=10.0, column_top=15000.0, mass_rate=1.0e6,
+                        sources=None, nsources=1, amp_factors=None, dtype_c=np.float64) -> Problem:
+    """Operational-size single domain (Fourier wind + shear, line source, log-spaced v_s).
+
+    Wind: FourierWind(seed) evaluated at each face centre; w = 0. Settling
+    velocities v_b = vmin*(vmax/vmin)^((b-1)/(nb-1)) on every w-face. Line
+    source(s): TOP-HAT-like uniform split of `mass_rate` over bins and over
+    the column cells whose centre lies below `column_top`
+    (Sources/mod_Src.F90:1406-1423 places a top-hat column; here each source
+    column sits in one cell column). With nsources > 1 the source columns
+    are drawn from the seeded generator. `sources` may override the list of
+    (i, j, column_top, mass_rate) tuples.
+    """
+    g = uniform_grid(nx, ny, nz, dx, dy, dz)
+    Lx, Ly = nx * dx, ny * dy
+    fw = FourierWind(seed, amp_factors=amp_factors)
+    fw.calibrate(Lx, Ly)
+    zk = (np.arange(1, nz + 1) - 0.5) * dz
+    # u at x-faces: x = (f-1) dx, y = (j-1/2) dy
+    xf = (np.arange(0, nx + 3) - 1.0) * dx
+    yc = (np.arange(1, ny + 1) - 0.5) * dy
+    X, Y = np.meshgrid(xf, yc, indexing="ij")
+    u2d, _ = fw.uv(X, Y)
+    u = _F((nx + 3, ny, nz))
+    u[...] = u2d[:, :, None] + (fw.shear * 1e-3) * zk[None, None, :]
+    # v at y-faces (v layout is y-fastest: v(f, i, k))
+    yf = (np.arange(0, ny + 3) - 1.0) * dy
+    xc = (np.arange(1, nx + 1) - 0.5) * dx
+    Yf, Xc = np.meshgrid(yf, xc, indexing="ij")
+    _, v2d = fw.uv(Xc, Yf)
+    v = _F((ny + 3, nx, nz))
+    v[...] = v2d[:, :, None]
+    w = _F((nz + 3, nx, ny))
+    k1, k2, k3 = constant_diffusion(nx, ny, nz, kh, kv)
+    if nb > 1:
+        vset = vmin * (vmax / vmin) ** (np.arange(nb) / (nb - 1))
+    else:
+        vset = np.array([vmin])
+    vs = _F((nx, ny, nz + 3, nb))
+    vs[...] = vset[None, None, None, :]
+    c = _F((nx + 4, ny + 4, nz + 4, nb))
+    s = _F((nx, ny, nz, nb))
+    if sources is None:
+        if nsources == 1:
+            sources = [(nx // 2, ny // 2, column_top, mass_rate)]
+        else:
+            rng = np.random.default_rng(seed + 7919)
+            sources = []
+            for _ in range(nsources):
+                i = int(rng.integers(nx // 5, 4 * nx // 5)) + 1
+                j = int(rng.integers(ny // 5, 4 * ny // 5)) + 1
+                sources.append((i, j, column_top, mass_rate))
+    for (i, j, top, rate) in sources:
+        ks = [k for k in range(1, nz + 1) if zk[k - 1] <= top]
+        per = rate / (nb * len(ks))
+        for k in ks:
+            s[i - 1, j - 1, k - 1, :] += per
+    meta = {"dx": dx, "dy": dy, "dz": dz, "kh": kh, "kv": kv, "seed": seed, "vset": vset.tolist(),
+            "sources": sources}
+    return Problem(nx, ny, nz, nb, g, u, u.copy(order="F"), v, v.copy(order="F"), w, w.copy(order="F"),
+                   k1, k2, k3, vs, c, s, name=f"oper{nx}x{ny}x{nz}x{nb}s{seed}", meta=meta)
+
+
+def random_problem(nx, ny, nz, nb, seed=0, dx=1000.0, dy=1200.0, dz=150.0, wscale=5.0, stretched=False,
+                   umax_scale=1.0) -> Problem:
+    """Small random problem exercising every branch (signs of every face velocity,
+    non-uniform diffusion, rough concentrations with exact zeros and plateaus
+    that hit the limiter's EPSILON branch). Used by the parity tests."""
+    rng = np.random.default_rng(seed)
+    if stretched:
+        s = np.concatenate([[0.0], np.cumsum(rng.uniform(0.5, 1.5, nz))])
+        s = s / s[-1]
+        g = sigma_grid(nx, ny, nz, dx, dy, s, nz * dz)
+    else:
+        g = uniform_grid(nx, ny, nz, dx, dy, dz)
+    u = np.asfortranarray(rng.normal(0.0, wscale * umax_scale, (nx + 3, ny, nz)))
+    v = np.asfortranarray(rng.normal(0.0, wscale * umax_scale, (ny + 3, nx, nz)))
+    w = np.asfortranarray(rng.normal(0.0, 0.3 * wscale * umax_scale, (nz + 3, nx, ny)))
+    w2 = np.asfortranarray(w + rng.normal(0.0, 0.1 * wscale, w.shape))
+    k1 = np.asfortranarray(rng.uniform(10.0, 200.0, (nx + 4, ny, nz)))
+    k2 = np.asfortranarray(rng.uniform(10.0, 200.0, (ny + 4, nx, nz)))
+    k3 = np.asfortranarray(rng.uniform(1.0, 20.0, (nz + 4, nx, ny)))
+    vs = np.asfortranarray(rng.uniform(0.0, 1.0, (nx, ny, nz + 3, nb)))
+    c = np.asfortranarray(rng.uniform(0.0, 1.0, (nx + 4, ny + 4, nz + 4, nb)))
+    # plateaus (EPSILON branch) and exact zeros
+    mask = rng.uniform(size=c.shape) < 0.15
+    c[mask] = 0.0
+    c[2:5, 2:5, :, :] = 0.25
+    s = np.asfortranarray(np.where(rng.uniform(size=(nx, ny, nz, nb)) < 0.05,
+                                   rng.uniform(0.0, 1e3, (nx, ny, nz, nb)), 0.0))
+    return Problem(nx, ny, nz, nb, g, u, u.copy(order="F"), v, v.copy(order="F"), w, w2, k1, k2, k3, vs, c, s,
+                   name=f"rand{nx}x{ny}x{nz}x{nb}s{seed}")

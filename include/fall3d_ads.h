@@ -1,0 +1,198 @@
+/*
+ * fall3d_ads.h -- C ABI of the MI355X (gfx950) ADS step library
+ * (libfall3d_ads_hip.so).
+ *
+ * The library replaces FALL3D's MODULE ADS (Sources/mod_ADS.F90) and the
+ * ADS part of the time step around it. Two groups of entry points:
+ *
+ *  1. Drop-in group: one C function per public procedure of MODULE ADS
+ *     (Sources/mod_ADS.F90:48-55), same argument meaning and order, host
+ *     arrays in the reference's layouts and kind (real(rp), rp = 4 or 8).
+ *     Each call copies its arrays to the GPU, runs the sweep there and
+ *     copies the results back (correct, but PCIe-bound). This is what
+ *     fall3d_amd/fortran/mod_ADS_hip.F90 binds through ISO_C_BINDING so
+ *     that Sources/mod_F3D.F90 compiles and runs unchanged.
+ *
+ *  2. Resident group (no counterpart procedure in the reference; it fuses
+ *     what Sources/mod_F3D.F90:1397-1491 does around the ADS calls): the
+ *     device owns the concentration field, the settling velocities and the
+ *     wind/diffusion snapshots; a whole F3D_time_step (source injection,
+ *     x/y sweeps in iiter-parity order, z sweep with ground deposition) is
+ *     enqueued by fall3d_ads_step without host traffic. Host copies happen
+ *     only at fall3d_ads_set_state / get_state / get_budget.
+ *
+ * Array layouts (Fortran column-major; bounds as in Sources/mod_Domain.F90
+ * for the whole single-GPU domain, ips=1, ipe=nx, 2 ghost cells, 1 halo face):
+ *   c    (-1:nx+2,-1:ny+2,-1:nz+2,nbins)   k1 (-1:nx+2,1:ny,1:nz)
+ *   u    ( 0:nx+2, 1:ny,   1:nz)           k2 (-1:ny+2,1:nx,1:nz)
+ *   v    ( 0:ny+2, 1:nx,   1:nz)           k3 (-1:nz+2,1:nx,1:ny)
+ *   w    ( 0:nz+2, 1:nx,   1:ny)           vs ( 1:nx,1:ny,0:nz+2,nbins)
+ *   acum (-1:nx+2,-1:ny+2,nbins)           s  ( 1:nx,1:ny,1:nz,nbins)
+ *   dX1_p(-1:nx+2) dX1_b(0:nx+2), likewise dX2_*, dX3_*
+ * Every array pointer is a HOST pointer of kind real_kind (4 or 8 bytes)
+ * unless stated otherwise; flux accumulators are always double.
+ *
+ * All functions return FALL3D_ADS_OK (0) or a positive error code; the
+ * message is available from fall3d_ads_last_error. A context is not
+ * thread-safe (one host thread per context), as MODULE ADS is a singleton.
+ */
+#ifndef FALL3D_ADS_H
+#define FALL3D_ADS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fall3d_ads_ctx fall3d_ads_ctx;
+
+/* limiter (Sources/mod_ADS.F90:121-131) */
+#define FALL3D_ADS_MINMOD 1
+#define FALL3D_ADS_SUPERBEE 2
+#define FALL3D_ADS_OSPRE 3
+/* time marching (Sources/mod_ADS.F90:43-44) */
+#define FALL3D_ADS_EULER 1
+#define FALL3D_ADS_RK 2
+/* device state of the concentration: plain FP32, or FP32 hi + FP32 lo
+ * (compensated; telescoping flux updates are then exact to ~2^-48 and the
+ * mass budget closes to FP64 rounding). */
+#define FALL3D_ADS_STATE_FP32 0
+#define FALL3D_ADS_STATE_FP32X2 1
+
+/* status codes */
+#define FALL3D_ADS_OK 0
+#define FALL3D_ADS_EINVAL 1
+#define FALL3D_ADS_EHIP 2
+#define FALL3D_ADS_ESTATE 3
+
+/* budget vector of fall3d_ads_get_budget */
+#define FALL3D_ADS_BUDGET_MASS_VOLUME 0 /* sum c dX1_p dX2_p dX3_p (Sources/mod_Grid.F90:1364-1375) */
+#define FALL3D_ADS_BUDGET_MASS_CELLS 1  /* sum c dX1_b dX2_b dX3_b (measure conserved by the flux form) */
+#define FALL3D_ADS_BUDGET_GROUND 2      /* |D_flux| (Sources/mod_Grid.F90:1488) */
+#define FALL3D_ADS_BUDGET_LATERAL 3     /* |W|+|E|+|S|+|N|+|U| (Sources/mod_Grid.F90:1499-1500) */
+#define FALL3D_ADS_BUDGET_INJECTED 4    /* mass added by source injection */
+#define FALL3D_ADS_BUDGET_LEDGER_W 5    /* mass that left through W,E,S,N,D,U (RK-weighted, exact) */
+#define FALL3D_ADS_BUDGET_LEN 11
+
+/* ---------------------------------------------------------------------
+ * Drop-in group (MODULE ADS)
+ * --------------------------------------------------------------------- */
+
+/* ADS_initialize(limitertype, time_marching), Sources/mod_ADS.F90:98-157.
+ * The grid extents replace the Domain module globals the reference reads
+ * (my_ips..my_kpe, Sources/mod_Domain.F90:263-330); real_kind is the host
+ * real(rp) kind (4 or 8). state_mode selects the device state
+ * (FALL3D_ADS_STATE_*). */
+int fall3d_ads_initialize(fall3d_ads_ctx **ctx, int nx, int ny, int nz, int nbins, int limiter,
+                          int time_marching, int real_kind, int state_mode);
+
+/* The ARAKAWA_C_GRID spacing arrays ADS reads (Sources/mod_KindType.F90:486-491):
+ * dX1_p(-1:nx+2), dX1_b(0:nx+2), ... Must be called before any solve. */
+int fall3d_ads_set_grid(fall3d_ads_ctx *ctx, const void *dX1_p, const void *dX1_b, const void *dX2_p,
+                        const void *dX2_b, const void *dX3_p, const void *dX3_b);
+
+/* ADS_get_vcomps(my_u, my_v, my_w), Sources/mod_ADS.F90:166-177: returns
+ * the host arrays CB%u (0:nx+2,1:ny,1:nz), CB%v, CB%w that the caller fills
+ * before the sweeps (Sources/mod_F3D.F90:1435-1439). They are owned by the
+ * context and uploaded at the next solve. */
+int fall3d_ads_get_vcomps(fall3d_ads_ctx *ctx, void **u, void **v, void **w);
+
+/* ADS_update_horizontal_vcomps(u1,u2,v1,v2,stime), Sources/mod_ADS.F90:187-198:
+ * CB%u = (1-stime) u1 + stime u2, CB%v likewise (computed on the GPU). */
+int fall3d_ads_update_horizontal_vcomps(fall3d_ads_ctx *ctx, const void *u1, const void *u2, const void *v1,
+                                        const void *v2, double stime);
+
+/* ADS_solve_along_x(W_flux,E_flux,dt,c,k1,GRID,nbins), Sources/mod_ADS.F90:233-412.
+ * c is updated in place (interior and the free-flow x ghosts); W/E_flux are
+ * incremented by the boundary mass fluxes of the last RK stage. */
+int fall3d_ads_solve_along_x(fall3d_ads_ctx *ctx, double *W_flux, double *E_flux, double dt, void *c,
+                             const void *k1);
+
+/* ADS_solve_along_y(S_flux,N_flux,dt,c,k2,GRID,nbins), Sources/mod_ADS.F90:422-602. */
+int fall3d_ads_solve_along_y(fall3d_ads_ctx *ctx, double *S_flux, double *N_flux, double dt, void *c,
+                             const void *k2);
+
+/* ADS_solve_along_z(D_flux,U_flux,dt,c,acum,k3,w1,w2,stime,vs,GRID,nbins),
+ * Sources/mod_ADS.F90:612-811. CB%w is formed as (1-stime) w1 + stime w2 - vs
+ * per bin; acum += |ground flux| dt. */
+int fall3d_ads_solve_along_z(fall3d_ads_ctx *ctx, double *D_flux, double *U_flux, double dt, void *c,
+                             void *acum, const void *k3, const void *w1, const void *w2, double stime,
+                             const void *vs);
+
+/* ADS_solve_1D(c,dxp,dxb,u,k,dt,bvalue,flux,bcode,ips,ipe,ibs,ibe,limiter,idime),
+ * Sources/mod_ADS.F90:820-1053: one line integrated with RK4 and per-side
+ * boundary code bcode(2) in {0 none, -1 periodic, 1 Dirichlet, 2 free flow}
+ * (idime is accepted and ignored, as in the reference). Arrays:
+ * c(ips-2:ipe+2), dxp(ips-2:ipe+2), dxb(ibs-1:ibe+1), u(ibs-1:ibe+1),
+ * k(ips-2:ipe+2) of real_kind; flux(2) double. Runs on the GPU. */
+int fall3d_ads_solve_1d(fall3d_ads_ctx *ctx, void *c, const void *dxp, const void *dxb, const void *u,
+                        const void *k, double dt, const double bvalue[2], double flux[2], const int bcode[2],
+                        int ips, int ipe, int limiter);
+
+/* ADS_release(), Sources/mod_ADS.F90:208-224. Frees every device resource. */
+int fall3d_ads_release(fall3d_ads_ctx *ctx);
+
+/* Message of the last failing call on this context (never NULL). */
+const char *fall3d_ads_last_error(const fall3d_ads_ctx *ctx);
+
+/* ---------------------------------------------------------------------
+ * Resident group (fuses Sources/mod_F3D.F90:1397-1491 around the sweeps)
+ * --------------------------------------------------------------------- */
+
+/* Upload one meteo interval: the two snapshots of the scaled face winds
+ * MY_MET%my_u/v/w(...,1:2) and the diffusion/settling fields
+ * (Sources/mod_F3D.F90:916-944). Any of u2/v2/w2 may be NULL (= snapshot 1). */
+int fall3d_ads_set_meteo(fall3d_ads_ctx *ctx, const void *u1, const void *u2, const void *v1, const void *v2,
+                         const void *w1, const void *w2, const void *k1, const void *k2, const void *k3,
+                         const void *vs);
+
+/* Dense source rate s(1:nx,1:ny,1:nz,nbins) (MY_TRA%my_s); stored sparsely
+ * on the device (only nonzero entries are injected). NULL clears it. */
+int fall3d_ads_set_source(fall3d_ads_ctx *ctx, const void *s);
+
+/* Concentration (with ghosts) and ground accumulation; acum may be NULL. */
+int fall3d_ads_set_state(fall3d_ads_ctx *ctx, const void *c, const void *acum);
+int fall3d_ads_get_state(fall3d_ads_ctx *ctx, void *c, void *acum);
+
+/* grid_get_time_step (Sources/mod_Grid.F90:1026-1141) on the GPU with an
+ * FP64 max-reduction, for the winds interpolated at stime. criterion 1 or 2. */
+int fall3d_ads_get_time_step(fall3d_ads_ctx *ctx, double stime, int criterion, double cfl, double *dt);
+
+/* Enqueue one F3D_time_step (source add, sweeps x/y ordered by iiter parity,
+ * z) on the context stream. Asynchronous. */
+int fall3d_ads_step(fall3d_ads_ctx *ctx, double dt, double stime, int iiter);
+
+/* Enqueue nsteps steps starting at iteration iiter0 with fixed dt/stime. */
+int fall3d_ads_run(fall3d_ads_ctx *ctx, double dt, double stime, int iiter0, int nsteps);
+
+/* Reference flux accumulators W,E,S,N,D,U (FP64, device resident). */
+int fall3d_ads_get_fluxes(fall3d_ads_ctx *ctx, double flux[6]);
+int fall3d_ads_set_fluxes(fall3d_ads_ctx *ctx, const double flux[6]);
+
+/* FP64 budget, FALL3D_ADS_BUDGET_* indices (synchronizes). */
+int fall3d_ads_get_budget(fall3d_ads_ctx *ctx, double out[FALL3D_ADS_BUDGET_LEN]);
+
+/* Stream control: use an external hipStream_t (NULL = the library's own). */
+int fall3d_ads_set_stream(fall3d_ads_ctx *ctx, void *stream);
+int fall3d_ads_synchronize(fall3d_ads_ctx *ctx);
+
+/* Tuning: cells per thread (0 = automatic) and x-columns per workgroup for
+ * the sweep of axis dir (0 x, 1 y, 2 z); bins per workgroup. */
+int fall3d_ads_set_tiling(fall3d_ads_ctx *ctx, int dir, int cells_per_thread, int columns, int bins_per_group);
+
+/* Per-kernel timing with HIP events on the context stream.
+ * kernel ids: 0 source, 1 sweep x, 2 sweep y, 3 sweep z, 4 reduce. */
+int fall3d_ads_set_profiling(fall3d_ads_ctx *ctx, int on);
+int fall3d_ads_get_kernel_time(fall3d_ads_ctx *ctx, int kernel, double *total_ms, long long *launches);
+
+/* Algorithmic HBM bytes one launch of sweep `dir` moves (state + coefficient
+ * arrays it must read once), for roofline accounting. */
+int fall3d_ads_sweep_bytes(fall3d_ads_ctx *ctx, int dir, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FALL3D_ADS_H */

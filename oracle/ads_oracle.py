@@ -1,0 +1,166 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/ads_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker (never as the thing that
+is measured or shipped).
+
+All arrays are numpy float64 in the reference's Fortran layouts (order='F'),
+see the header of ads_oracle.c for the exact bounds of each array.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libads_oracle.so")
+_lib = None
+
+LIMITERS = {"minmod": 1, "superbee": 2, "ospre": 3}
+TIME_MARCHING = {"euler": 1, "rk": 2}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build() -> str:
+    """Compile the oracle with the committed Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        for name in ("ora_solve_x", "ora_solve_y"):
+            f = getattr(L, name)
+            f.argtypes = [_ip, ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p]
+            f.restype = None
+        L.ora_solve_z.argtypes = [_ip, ctypes.c_void_p, ctypes.c_double, _dp, ctypes.c_void_p, _dp, _dp, _dp,
+                                  ctypes.c_double, _dp, _dp, _dp, ctypes.c_void_p]
+        L.ora_solve_z.restype = None
+        L.ora_add_source.argtypes = [_ip, ctypes.c_void_p, ctypes.c_double, _dp, _dp]
+        L.ora_add_source.restype = ctypes.c_double
+        L.ora_time_step.argtypes = [_ip, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                    ctypes.c_double]
+        L.ora_time_step.restype = ctypes.c_double
+        L.ora_mass_volume.argtypes = [_ip, ctypes.c_void_p, _dp]
+        L.ora_mass_volume.restype = ctypes.c_double
+        L.ora_mass_cells.argtypes = [_ip, ctypes.c_void_p, _dp]
+        L.ora_mass_cells.restype = ctypes.c_double
+        L.ora_solve_1d.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _ip,
+                                   ctypes.c_int]
+        L.ora_solve_1d.restype = None
+        L.ora_num_threads.argtypes = []
+        L.ora_num_threads.restype = ctypes.c_int
+        L.ora_set_num_threads.argtypes = [ctypes.c_int]
+        L.ora_set_num_threads.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "float64 Fortran-ordered array expected"
+    return a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    """FP64 oracle bound to one problem geometry (a `fall3d_amd.problem.Problem`)."""
+
+    def __init__(self, prob, limiter="minmod", time_marching="rk", boundaries=(1, 1, 1, 1, 1, 1)):
+        self.prob = prob
+        cfg = [prob.nx, prob.ny, prob.nz, prob.nb, *boundaries,
+               LIMITERS[limiter] if isinstance(limiter, str) else int(limiter),
+               TIME_MARCHING[time_marching] if isinstance(time_marching, str) else int(time_marching)]
+        self._cfg = (ctypes.c_int * len(cfg))(*cfg)
+        g = prob.grid
+        self._gridarrs = [np.ascontiguousarray(g[k], dtype=np.float64)
+                          for k in ("dX1_p", "dX1_b", "dX2_p", "dX2_b", "dX3_p", "dX3_b")]
+        self._grid = (ctypes.c_void_p * 6)(*[a.ctypes.data for a in self._gridarrs])
+        self.flux = np.zeros(6)      # W,E,S,N,D,U (reference accumulators)
+        self.ledger = np.zeros(6)    # conservation ledger (mass out per side)
+
+    # --- sweeps (in place on c/acum) ------------------------------------
+    def solve_x(self, dt, c, k1, u):
+        W, E = ctypes.c_double(self.flux[0]), ctypes.c_double(self.flux[1])
+        led = np.zeros(2)
+        lib().ora_solve_x(self._cfg, self._grid, dt, _p(c), _p(k1), _p(u), ctypes.byref(W), ctypes.byref(E),
+                          led.ctypes.data)
+        self.flux[0], self.flux[1] = W.value, E.value
+        self.ledger[0:2] += led
+
+    def solve_y(self, dt, c, k2, v):
+        S, N = ctypes.c_double(self.flux[2]), ctypes.c_double(self.flux[3])
+        led = np.zeros(2)
+        lib().ora_solve_y(self._cfg, self._grid, dt, _p(c), _p(k2), _p(v), ctypes.byref(S), ctypes.byref(N),
+                          led.ctypes.data)
+        self.flux[2], self.flux[3] = S.value, N.value
+        self.ledger[2:4] += led
+
+    def solve_z(self, dt, c, acum, k3, w1, w2, stime, vs):
+        D, U = ctypes.c_double(self.flux[4]), ctypes.c_double(self.flux[5])
+        led = np.zeros(2)
+        lib().ora_solve_z(self._cfg, self._grid, dt, _p(c), acum.ctypes.data if acum is not None else None,
+                          _p(k3), _p(w1), _p(w2), stime, _p(vs), ctypes.byref(D), ctypes.byref(U),
+                          led.ctypes.data)
+        self.flux[4], self.flux[5] = D.value, U.value
+        self.ledger[4:6] += led
+
+    def add_source(self, dt, c, s):
+        return lib().ora_add_source(self._cfg, self._grid, dt, _p(c), _p(s))
+
+    def time_step(self, u, v, w, vs, criterion=2, cfl=0.9):
+        p = self.prob
+        return lib().ora_time_step(self._cfg, self._grid, _p(u), _p(v), _p(w), _p(p.k1), _p(p.k2), _p(p.k3),
+                                   _p(vs), criterion, cfl)
+
+    def mass_volume(self, c):
+        return lib().ora_mass_volume(self._cfg, self._grid, _p(c))
+
+    def mass_cells(self, c):
+        return lib().ora_mass_cells(self._cfg, self._grid, _p(c))
+
+    # --- one F3D_time_step with steady winds (mod_F3D.F90:1437-1491) -------
+    def step(self, dt, iiter, c, acum, stime=0.0, source=True):
+        p = self.prob
+        u = (1.0 - stime) * p.u1 + stime * p.u2 if stime else p.u1
+        v = (1.0 - stime) * p.v1 + stime * p.v2 if stime else p.v1
+        inj = 0.0
+        if source and p.s is not None:
+            inj = self.add_source(dt, c, p.s)
+        u = np.asfortranarray(u)
+        v = np.asfortranarray(v)
+        if iiter % 2 == 0:
+            self.solve_x(dt, c, p.k1, u)
+            self.solve_y(dt, c, p.k2, v)
+        else:
+            self.solve_y(dt, c, p.k2, v)
+            self.solve_x(dt, c, p.k1, u)
+        self.solve_z(dt, c, acum, p.k3, p.w1, p.w2, stime, p.vs)
+        return inj
+
+
+def solve_1d(c, dxp, dxb, u, k, dt, bvalue, bcode, limiter):
+    """ADS_solve_1D on one line (c updated in place); returns flux(2)."""
+    n = c.size - 4
+    bv = np.asarray(bvalue, dtype=np.float64)
+    fl = np.zeros(2)
+    bc = (ctypes.c_int * 2)(*bcode)
+    for a in (c, dxp, dxb, u, k):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    lib().ora_solve_1d(n, _p(c), _p(dxp), _p(dxb), _p(u), _p(k), float(dt), _p(bv), _p(fl), bc, int(limiter))
+    return fl[0], fl[1]
+
+
+def set_num_threads(n: int) -> None:
+    lib().ora_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return lib().ora_num_threads()
