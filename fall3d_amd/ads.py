@@ -48,6 +48,7 @@ EXPORTS = (
     "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
+    "fall3d_ads_set_graph",
 )
 
 
@@ -98,6 +99,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_set_profiling": ([_vp, _int], _int),
         "fall3d_ads_get_kernel_time": ([_vp, _int, _dp, ctypes.POINTER(ctypes.c_longlong)], _int),
         "fall3d_ads_sweep_bytes": ([_vp, _int, _dp], _int),
+        "fall3d_ads_set_graph": ([_vp, _int], _int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -351,6 +353,9 @@ class ADS:
     def set_tiling(self, axis, cells_per_thread=0, columns=0, bins_per_group=0):
         self._chk(load_library().fall3d_ads_set_tiling(self._h, int(axis), int(cells_per_thread), int(columns),
                                                        int(bins_per_group)), "set_tiling")
+
+    def set_graph(self, on=True):
+        self._chk(load_library().fall3d_ads_set_graph(self._h, 1 if on else 0), "set_graph")
 
     def set_profiling(self, on=True):
         self._chk(load_library().fall3d_ads_set_profiling(self._h, 1 if on else 0), "set_profiling")

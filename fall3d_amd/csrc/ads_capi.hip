@@ -93,6 +93,13 @@ struct fall3d_ads_ctx {
   Tiling tl[3];
   int user_cpt[3] = {0, 0, 0}, user_tx[3] = {0, 0, 0}, user_bpg[3] = {0, 0, 0};
 
+  // graph replay of step pairs (even + odd iteration)
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = -1.0, g_stime = -1.0;
+  int g_version = -1;
+  int version = 0;  // bumped whenever launch parameters of a step change
+
   // profiling
   bool prof = false;
   std::vector<PendingEvent> pending;
@@ -194,11 +201,12 @@ const int kCpt[] = {2, 3, 4, 6, 8, 12, 16};
 Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
   Tiling t;
   const int n = axis_n(c, dir);
-  const int ucpt = c->user_cpt[dir], utx = c->user_tx[dir];
   if (dir == 0) {
+    // one wave per line (several short lines per wave), C cells per lane,
+    // all bins of a line in the same wave unless the grid is too small
     int C = 16;
-    if (ucpt) {
-      C = ucpt;
+    if (c->user_cpt[0]) {
+      C = c->user_cpt[0];
     } else {
       for (int cc : kCpt) {
         if ((n + cc - 1) / cc <= 64) { C = cc; break; }
@@ -206,69 +214,50 @@ Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
     }
     t.cpt = C;
     t.S = (n + C - 1) / C;
+    if (t.S > 64) t.S = 0;  // unsupported; caught by the caller
     t.block = 256;
-    if (t.S > 256) t.S = 0;  // unsupported; caught by the caller
-    const int lpb = t.S ? 256 / t.S : 1;
-    t.TX = lpb;
+    const int lpw = t.S ? 64 / t.S : 1;
+    t.TX = 4 * lpw;  // lines per workgroup
     const long nlines = (long)c->ny * c->nz;
-    t.gx = (int)((nlines + lpb - 1) / lpb);
+    t.gx = (int)((nlines + t.TX - 1) / t.TX);
+    int groups = 1;
+    if (c->user_bpg[0] > 0) {
+      groups = (c->nb + c->user_bpg[0] - 1) / c->user_bpg[0];
+    } else {
+      const long waves = (long)t.gx * 4;
+      const long target = 4096;
+      if (waves < target) groups = (int)std::min<long>(c->nb, (target + waves - 1) / std::max<long>(waves, 1));
+    }
+    t.bpg = (c->nb + groups - 1) / groups;
+    t.groups = (c->nb + t.bpg - 1) / t.bpg;
     t.ntrans = 0;
   } else {
-    int bestC = 0, bestTX = 0;
-    if (ucpt || utx) {
-      bestC = ucpt ? ucpt : 8;
-      bestTX = utx ? utx : 64;
-    } else {
-      const int txs[3] = {64, 32, 16};
-      for (int tx : txs) {
-        for (int cc : {2, 3, 4, 6, 8}) {
-          if (((n + cc - 1) / cc) * tx <= 512) { bestC = cc; bestTX = tx; break; }
-        }
-        if (bestC) break;
-      }
-      if (!bestC) {
-        const int txs2[4] = {32, 16, 8, 4};
-        for (int tx : txs2) {
-          for (int cc : {12, 16}) {
-            if (((n + cc - 1) / cc) * tx <= 512) { bestC = cc; bestTX = tx; break; }
-          }
-          if (bestC) break;
-        }
-      }
-    }
-    t.cpt = bestC;
-    t.TX = bestTX;
-    t.S = bestC ? (n + bestC - 1) / bestC : 0;
-    t.block = t.S * t.TX;
-    if (t.block > 512 || t.block <= 0) t.S = 0;
-    t.ntrans = (c->nx + bestTX - 1) / (bestTX ? bestTX : 1);
-    t.gx = t.ntrans * (dir == 1 ? c->nz : c->ny);
+    // streaming kernel: lanes = 64 x-columns, waves = bins of those columns
+    int bpg = c->user_bpg[dir] > 0 ? std::min(c->user_bpg[dir], 8) : 8;
+    int groups = (c->nb + bpg - 1) / bpg;
+    bpg = (c->nb + groups - 1) / groups;
+    t.cpt = 1;
+    t.S = 1;
+    t.TX = 64;
+    t.bpg = bpg;
+    t.groups = groups;
+    t.block = 64 * bpg;
+    t.ntrans = (c->nx + 63) / 64;
+    t.gx = groups * t.ntrans * (dir == 1 ? c->nz : c->ny);
   }
-  // bins per workgroup: all bins (coefficients read once) unless the grid
-  // would leave the chip underfilled
-  const int waves_per_wg = (t.block + 63) / 64;
-  long waves = (long)t.gx * waves_per_wg;
-  int groups = 1;
-  if (c->user_bpg[dir] > 0) {
-    groups = (c->nb + c->user_bpg[dir] - 1) / c->user_bpg[dir];
-  } else {
-    const long target = 8192;
-    if (waves < target) groups = (int)std::min<long>(c->nb, (target + waves - 1) / std::max<long>(waves, 1));
-    if (groups < 1) groups = 1;
-  }
-  t.bpg = (c->nb + groups - 1) / groups;
-  t.groups = (c->nb + t.bpg - 1) / t.bpg;
+  (void)n;
   return t;
 }
 
 int refresh_tiling(fall3d_ads_ctx *c) {
+  c->version++;
   for (int d = 0; d < 3; ++d) {
     Tiling t = choose_tiling(c, d);
-    if (t.S <= 0 || t.cpt < 2 || !sweep_cpt_supported(t.cpt))
-      return fail(c, FALL3D_ADS_EINVAL, "no supported tiling for axis %d (n=%d cpt=%d tx=%d)", d, axis_n(c, d),
-                  t.cpt, t.TX);
+    if (d == 0 && (t.S <= 0 || t.cpt < 2 || !sweep_cpt_supported(t.cpt)))
+      return fail(c, FALL3D_ADS_EINVAL, "no supported x tiling (nx=%d cpt=%d; nx <= 1024 needs cpt <= 16)",
+                  axis_n(c, d), t.cpt);
     c->tl[d] = t;
-    const size_t need = (size_t)t.gx * t.groups;
+    const size_t need = (d == 0) ? (size_t)t.gx * t.groups : (size_t)t.gx;
     if (c->part_cap[d + 1] < need) {
       dfree(c->part[d + 1]);
       HIPCHK(c, dalloc(&c->part[d + 1], need * 4));
@@ -297,6 +286,7 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   a.S = t.S;
   a.TX = t.TX;
   a.ntrans = t.ntrans;
+  a.ntrans2 = t.groups;
   a.bin0 = 0;
   a.nbin = t.bpg;
   a.nbins = c->nb;
@@ -323,14 +313,15 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   }
   a.acum = c->acum;
   a.partials = c->part[dir + 1];
-  dim3 grid(t.gx, t.groups), block(t.block);
-  const size_t shmem = sizeof(float) * 8 * (size_t)t.block;
+  dim3 grid = (dir == 0) ? dim3(t.gx, t.groups) : dim3(t.gx);
+  dim3 block(t.block);
   {
     ProfScope ps(c, KID_SWEEP_X + dir);
-    hipError_t e = launch_sweep(dir, t.cpt, c->limiter, c->comp != 0, a, grid, block, shmem, c->stream);
+    hipError_t e = launch_sweep(dir, t.cpt, c->limiter, c->comp != 0, c->tm == 1 ? 1 : 4, a, grid, block,
+                                c->stream);
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "sweep %d launch: %s", dir, hipGetErrorString(e));
   }
-  out->nblocks = t.gx * t.groups;
+  out->nblocks = (dir == 0) ? t.gx * t.groups : t.gx;
   return FALL3D_ADS_OK;
 }
 
@@ -615,6 +606,7 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->evpool) (void)hipEventDestroy(e);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   void *ptrs[] = {c->chi, c->clo, c->acum, c->u1, c->u2, c->v1, c->v2, c->w1, c->w2, c->k1, c->k2, c->k3, c->vs,
                   c->src_idx, c->src_rate, c->src_meas, c->src_part, c->acc, c->part[1], c->part[2],
                   c->part[3], c->red_part, c->dstage};  // part[0] aliases src_part
@@ -822,6 +814,7 @@ int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
   c->src_rate = c->src_meas = c->src_part = nullptr;
   c->part[0] = nullptr;
   c->nsrc = 0;
+  c->version++;
   if (!s) return FALL3D_ADS_OK;
   const int nx = c->nx, ny = c->ny, nz = c->nz;
   std::vector<size_t> idx;
@@ -935,10 +928,58 @@ int fall3d_ads_step(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
 }
 
 int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int nsteps) {
-  for (int s = 0; s < nsteps; ++s) {
-    int rc = fall3d_ads_step(c, dt, stime, iiter0 + s);
-    if (rc) return rc;
+  if (!c) return FALL3D_ADS_EINVAL;
+  int rc;
+  int it = iiter0, left = nsteps;
+  // Launch-bound inner loop: replay a captured (even, odd) step pair as one
+  // hipGraph. Per-kernel timing needs events between kernels, so profiling
+  // runs launch directly.
+  const bool graph = c->use_graph && !c->prof && nsteps >= 4;
+  if (graph) {
+    if (it % 2 != 0) {
+      if ((rc = fall3d_ads_step(c, dt, stime, it))) return rc;
+      ++it;
+      --left;
+    }
+    if (!c->gexec || c->g_dt != dt || c->g_stime != stime || c->g_version != c->version) {
+      if (c->gexec) {
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+      }
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int r0 = fall3d_ads_step(c, dt, stime, 0);
+      int r1 = r0 ? r0 : fall3d_ads_step(c, dt, stime, 1);
+      hipGraph_t gr = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &gr);
+      if (r1) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return r1;
+      }
+      if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "step capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, FALL3D_ADS_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+      }
+      c->g_dt = dt;
+      c->g_stime = stime;
+      c->g_version = c->version;
+    }
+    while (left >= 2) {
+      HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      it += 2;
+      left -= 2;
+    }
   }
+  for (; left > 0; --left, ++it)
+    if ((rc = fall3d_ads_step(c, dt, stime, it))) return rc;
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_set_graph(fall3d_ads_ctx *c, int on) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  c->use_graph = on != 0;
   return FALL3D_ADS_OK;
 }
 

@@ -3,14 +3,14 @@
 
 namespace f3d {
 
-hipError_t launch_sweep_x(int cpt, int limiter, bool comp, const SweepArgs &a, dim3 grid, dim3 block, size_t shmem,
+hipError_t launch_sweep_x(int cpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,
                           hipStream_t st);
-hipError_t launch_sweep_y(int cpt, int limiter, bool comp, const SweepArgs &a, dim3 grid, dim3 block, size_t shmem,
+hipError_t launch_sweep_y(int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,
                           hipStream_t st);
-hipError_t launch_sweep_z(int cpt, int limiter, bool comp, const SweepArgs &a, dim3 grid, dim3 block, size_t shmem,
+hipError_t launch_sweep_z(int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,
                           hipStream_t st);
 
-// cells per thread with an instantiated kernel (see F3D_CPT_LIST)
+// cells per lane with an instantiated x-sweep kernel (see F3D_CPT_LIST)
 const int kSweepCpt[] = {2, 3, 4, 6, 8, 12, 16};
 const int kSweepCptCount = 7;
 
@@ -20,11 +20,11 @@ bool sweep_cpt_supported(int cpt) {
   return false;
 }
 
-hipError_t launch_sweep(int dir, int cpt, int limiter, bool comp, const SweepArgs &a, dim3 grid, dim3 block,
-                        size_t shmem, hipStream_t st) {
-  if (dir == 0) return launch_sweep_x(cpt, limiter, comp, a, grid, block, shmem, st);
-  if (dir == 1) return launch_sweep_y(cpt, limiter, comp, a, grid, block, shmem, st);
-  if (dir == 2) return launch_sweep_z(cpt, limiter, comp, a, grid, block, shmem, st);
+hipError_t launch_sweep(int dir, int cpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid,
+                        dim3 block, hipStream_t st) {
+  if (dir == 0) return launch_sweep_x(cpt, limiter, comp, nst, a, grid, block, st);
+  if (dir == 1) return launch_sweep_y(limiter, comp, nst, a, grid, block, st);
+  if (dir == 2) return launch_sweep_z(limiter, comp, nst, a, grid, block, st);
   return hipErrorInvalidValue;
 }
 

@@ -26,7 +26,8 @@ struct SweepArgs {
   int n;                  // cells along the sweep axis
   int S;                  // segments (threads) per line
   int TX;                 // lines per workgroup (x-columns for y/z sweeps)
-  int ntrans;             // lines along the first transverse axis handled by the grid
+  int ntrans;             // x tiles of 64 columns (y, z sweeps)
+  int ntrans2;            // bin groups (y, z sweeps; blockIdx.x = (line, tile) * groups + group)
   int bin0, nbin;         // bins handled per workgroup (blockIdx.y selects the group)
   int nbins;              // total bins
   int tm;                 // 1 Euler, 2 RK4
@@ -45,8 +46,8 @@ struct SweepArgs {
 };
 
 // Sweep launchers (ads_sweep_*.hip). cpt = cells per thread.
-hipError_t launch_sweep(int dir, int cpt, int limiter, bool comp, const SweepArgs &a, dim3 grid, dim3 block,
-                        size_t shmem, hipStream_t st);
+hipError_t launch_sweep(int dir, int cpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid,
+                        dim3 block, hipStream_t st);
 bool sweep_cpt_supported(int cpt);
 extern const int kSweepCpt[];
 extern const int kSweepCptCount;

@@ -1,29 +1,22 @@
-// ads_sweep_z.hip -- instantiations of the sweep kernel along z
+// ads_sweep_z.hip -- instantiations of the streaming z sweep
 // (Sources/mod_ADS.F90 ADS_solve_along_z); see ads_sweep_impl.h.
 #include "ads_sweep_impl.h"
 
 namespace f3d {
 
-#define F3D_CPT_LIST(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16)
-
-hipError_t launch_sweep_z(int cpt, int limiter, bool comp, const SweepArgs &a, dim3 grid, dim3 block,
-                            size_t shmem, hipStream_t st) {
-#define F3D_CASE(CPT)                                                                              \
-  if (cpt == CPT) {                                                                                \
-    if (!comp) {                                                                                   \
-      if (limiter == 1) { hipLaunchKernelGGL((k_sweep<2, CPT, 1, false>), grid, block, shmem, st, a); return hipGetLastError(); } \
-      if (limiter == 2) { hipLaunchKernelGGL((k_sweep<2, CPT, 2, false>), grid, block, shmem, st, a); return hipGetLastError(); } \
-      if (limiter == 3) { hipLaunchKernelGGL((k_sweep<2, CPT, 3, false>), grid, block, shmem, st, a); return hipGetLastError(); } \
-    } else {                                                                                       \
-      if (limiter == 1) { hipLaunchKernelGGL((k_sweep<2, CPT, 1, true>), grid, block, shmem, st, a); return hipGetLastError(); } \
-      if (limiter == 2) { hipLaunchKernelGGL((k_sweep<2, CPT, 2, true>), grid, block, shmem, st, a); return hipGetLastError(); } \
-      if (limiter == 3) { hipLaunchKernelGGL((k_sweep<2, CPT, 3, true>), grid, block, shmem, st, a); return hipGetLastError(); } \
-    }                                                                                              \
-    return hipErrorInvalidValue;                                                                   \
-  }
+hipError_t launch_sweep_z(int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,
+                            hipStream_t st) {
   (void)hipGetLastError();
-  F3D_CPT_LIST(F3D_CASE)
-#undef F3D_CASE
+#define F3D_L(L, CP, N)                                                                     \
+  if (limiter == L && comp == CP && nst == N) {                                             \
+    hipLaunchKernelGGL((k_stream<2, L, CP, N>), grid, block, 0, st, a);                  \
+    return hipGetLastError();                                                               \
+  }
+  F3D_L(1, false, 4) F3D_L(2, false, 4) F3D_L(3, false, 4)
+  F3D_L(1, true, 4) F3D_L(2, true, 4) F3D_L(3, true, 4)
+  F3D_L(1, false, 1) F3D_L(2, false, 1) F3D_L(3, false, 1)
+  F3D_L(1, true, 1) F3D_L(2, true, 1) F3D_L(3, true, 1)
+#undef F3D_L
   return hipErrorInvalidValue;
 }
 

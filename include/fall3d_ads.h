@@ -164,8 +164,14 @@ int fall3d_ads_get_time_step(fall3d_ads_ctx *ctx, double stime, int criterion, d
  * z) on the context stream. Asynchronous. */
 int fall3d_ads_step(fall3d_ads_ctx *ctx, double dt, double stime, int iiter);
 
-/* Enqueue nsteps steps starting at iteration iiter0 with fixed dt/stime. */
+/* Enqueue nsteps steps starting at iteration iiter0 with fixed dt/stime.
+ * Step pairs are replayed from a captured hipGraph (re-captured when dt,
+ * stime, the source or the tiling change) unless graphs are switched off
+ * or profiling is on. */
 int fall3d_ads_run(fall3d_ads_ctx *ctx, double dt, double stime, int iiter0, int nsteps);
+
+/* Use (1, default) or bypass (0) graph replay in fall3d_ads_run. */
+int fall3d_ads_set_graph(fall3d_ads_ctx *ctx, int on);
 
 /* Reference flux accumulators W,E,S,N,D,U (FP64, device resident). */
 int fall3d_ads_get_fluxes(fall3d_ads_ctx *ctx, double flux[6]);
