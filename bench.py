@@ -38,7 +38,24 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 PMC summary (FETCH/WRITE KiB per dispatch) for the traffic field")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates the FETCH_SIZE / "
+                         "WRITE_SIZE counters against a kernel of known traffic) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """Known-traffic reference for the HBM counters: each copy reads and
+    writes exactly mib MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
 
 
 def main():
@@ -47,6 +64,9 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU decomposition is a design (see the sharding notes), not a run.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     import numpy as np
     import torch
 

@@ -127,6 +127,22 @@ hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, h
   return hipGetLastError();
 }
 
+// dt/dxb(p) and 0.5 dt/dxb(p) in FP32 for the RK stage updates of one axis
+__global__ void k_dt_over_dxb(const double *__restrict__ dxb, int n3, double dt, float *__restrict__ dtib,
+                              float *__restrict__ hdtib) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n3) {
+    dtib[p] = (float)(dt / dxb[p]);
+    hdtib[p] = (float)(0.5 * dt / dxb[p]);
+  }
+}
+
+hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, float *dtib, float *hdtib, hipStream_t st) {
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_dt_over_dxb, dim3((n3 + 255) / 256), dim3(256), 0, st, dxb64, n3, dt, dtib, hdtib);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // source injection (Sources/mod_F3D.F90:1456-1473), sparse entries
 // ---------------------------------------------------------------------------

@@ -68,6 +68,10 @@ struct fall3d_ads_ctx {
   double *dxb64[3] = {nullptr, nullptr, nullptr};
   float *hdxp[3] = {nullptr, nullptr, nullptr};
   float *idxb[3] = {nullptr, nullptr, nullptr};
+  float *dtib[3] = {nullptr, nullptr, nullptr};   // dt/dxb for dtib_dt
+  float *hdtib[3] = {nullptr, nullptr, nullptr};  // 0.5 dt/dxb
+  double dtib_dt = 0.0;
+  bool dtib_ok = false;
   std::vector<double> hdxp64[3], hdxb64[3];
   bool grid_ok = false;
 
@@ -272,8 +276,25 @@ struct SweepOut {
   int nblocks = 0;
 };
 
+// dt/dxb tables of the three axes for this dt, refreshed (on the stream)
+// only when dt or the grid changes.
+int ensure_dt(fall3d_ads_ctx *c, double dt) {
+  if (c->dtib_ok && c->dtib_dt == dt) return FALL3D_ADS_OK;
+  for (int d = 0; d < 3; ++d) {
+    hipError_t e = dt_over_dxb(c->dxb64[d], axis_n(c, d) + 3, dt, c->dtib[d], c->hdtib[d], c->stream);
+    if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "dt/dxb: %s", hipGetErrorString(e));
+  }
+  c->dtib_dt = dt;
+  c->dtib_ok = true;
+  return FALL3D_ADS_OK;
+}
+
 int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *vel1, const float *vel2,
               const float *kd, const float *vsd, int write_ghosts, SweepOut *out) {
+  {
+    const int rc = ensure_dt(c, dt);
+    if (rc) return rc;
+  }
   const Tiling &t = c->tl[dir];
   SweepArgs a;
   memset(&a, 0, sizeof(a));
@@ -287,6 +308,7 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   a.TX = t.TX;
   a.ntrans = t.ntrans;
   a.ntrans2 = t.groups;
+  a.gx = t.gx;
   a.bin0 = 0;
   a.nbin = t.bpg;
   a.nbins = c->nb;
@@ -300,6 +322,8 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   a.vs = vsd;
   a.hdxp = c->hdxp[dir];
   a.idxb = c->idxb[dir];
+  a.dtib = c->dtib[dir];
+  a.hdtib = c->hdtib[dir];
   a.dxb64 = c->dxb64[dir];
   if (dir == 0) {
     a.areaA = c->dxb64[1];
@@ -586,6 +610,8 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     HIPCHK(c, dalloc(&c->dxb64[d], n + 3));
     HIPCHK(c, dalloc(&c->hdxp[d], n + 3));
     HIPCHK(c, dalloc(&c->idxb[d], n + 3));
+    HIPCHK(c, dalloc(&c->dtib[d], n + 3));
+    HIPCHK(c, dalloc(&c->hdtib[d], n + 3));
   }
   const size_t ksz = (size_t)real_kind;
   c->hu = calloc(n_u(c), ksz);
@@ -616,6 +642,8 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
     dfree(c->dxb64[d]);
     dfree(c->hdxp[d]);
     dfree(c->idxb[d]);
+    dfree(c->dtib[d]);
+    dfree(c->hdtib[d]);
   }
   free(c->hu);
   free(c->hv);
@@ -654,6 +682,7 @@ int fall3d_ads_set_grid(fall3d_ads_ctx *c, const void *dX1_p, const void *dX1_b,
     c->hdxb64[d] = b;
   }
   c->grid_ok = true;
+  c->dtib_ok = false;
   return FALL3D_ADS_OK;
 }
 
@@ -936,6 +965,9 @@ int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int n
   // runs launch directly.
   const bool graph = c->use_graph && !c->prof && nsteps >= 4;
   if (graph) {
+    // the dt/dxb tables are refreshed outside the graph (they may be stale
+    // after fall3d_ads_set_grid even when the graph itself is reused)
+    if ((rc = ensure_dt(c, dt))) return rc;
     if (it % 2 != 0) {
       if ((rc = fall3d_ads_step(c, dt, stime, it))) return rc;
       ++it;

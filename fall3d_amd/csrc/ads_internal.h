@@ -28,6 +28,7 @@ struct SweepArgs {
   int TX;                 // lines per workgroup (x-columns for y/z sweeps)
   int ntrans;             // x tiles of 64 columns (y, z sweeps)
   int ntrans2;            // bin groups (y, z sweeps; blockIdx.x = (line, tile) * groups + group)
+  int gx;                 // workgroups along the lines (x sweep: block = (bx, bin group))
   int bin0, nbin;         // bins handled per workgroup (blockIdx.y selects the group)
   int nbins;              // total bins
   int tm;                 // 1 Euler, 2 RK4
@@ -38,6 +39,8 @@ struct SweepArgs {
   const float *vs;           // settling velocity (z sweep only)
   const float *hdxp;         // 0.5/dxp(f-1) for faces f = 0..n+2
   const float *idxb;         // 1/dxb(p) for cells p = 0..n+2
+  const float *dtib;         // dt/dxb(p), p = 0..n+2 (for the launch's dt)
+  const float *hdtib;        // 0.5*dt/dxb(p)
   const double *dxb64;       // dxb(p) in FP64, p = 0..n+2 (mass measure of the ledger)
   const double *areaA;       // FP64 transverse cell sizes for the face area (index 0..)
   const double *areaB;
@@ -45,9 +48,22 @@ struct SweepArgs {
   double *partials;          // [gridDim.x*gridDim.y][4]: ref flux beg/end, ledger beg/end
 };
 
+// One launch over the members of an ensemble: member m owns the workgroups
+// bstart[m] .. bstart[m+1]-1 and is described by args[m] (device memory).
+struct SweepBatch {
+  const SweepArgs *args;
+  const int *bstart;  // nmem + 1 entries
+  int nmem;
+};
+
+// dt/dxb and 0.5 dt/dxb for one axis (ads_aux.hip)
+hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, float *dtib, float *hdtib, hipStream_t st);
+
 // Sweep launchers (ads_sweep_*.hip). cpt = cells per thread.
 hipError_t launch_sweep(int dir, int cpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid,
                         dim3 block, hipStream_t st);
+hipError_t launch_sweep_batch(int dir, int cpt, int limiter, bool comp, int nst, const SweepBatch &b,
+                              int nblocks, dim3 block, hipStream_t st);
 bool sweep_cpt_supported(int cpt);
 extern const int kSweepCpt[];
 extern const int kSweepCptCount;

@@ -34,6 +34,8 @@
 // (:394-396, :788-795); the conservation ledger uses Gh.
 #pragma once
 
+#include <type_traits>
+
 #include "ads_internal.h"
 
 namespace f3d {
@@ -104,44 +106,79 @@ __device__ __forceinline__ void block_partials(double v0, double v1, double v2, 
   }
 }
 
+typedef __attribute__((address_space(4))) const int cint;  // scalar-loaded tables
+
+// Member of an ensemble launch that owns workgroup bid: the last m with
+// bstart[m] <= bid (uniform, scalar loads).
+__device__ __forceinline__ int batch_member(const SweepBatch &b, const int bid) {
+  cint *bs = (cint *)b.bstart;
+  int lo = 0, hi = b.nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bs[mid] <= bid)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Arguments of member m, word by word through scalar loads.
+__device__ __forceinline__ SweepArgs member_args(const SweepBatch &b, const int m) {
+  static_assert(sizeof(SweepArgs) % 4 == 0, "SweepArgs is copied as 32-bit words");
+  constexpr int nw = (int)(sizeof(SweepArgs) / 4);
+  cint *src = (cint *)b.args + (size_t)m * nw;
+  SweepArgs a;
+  int *dst = reinterpret_cast<int *>(&a);
+#pragma unroll
+  for (int w = 0; w < nw; ++w) dst[w] = src[w];
+  return a;
+}
+
 // =============================================================================
 // k_stream: y (DIR 1) and z (DIR 2) sweeps, one thread per line
 // =============================================================================
 
-// Raw buffer access with hardware range checking: a load outside
-// [0, bytes) returns 0 and a store there is dropped, so the pipeline body
-// needs no branches for edge cells, idle lanes or overshooting iterations.
+// Raw buffer access. The per-lane part of an address goes in the VGPR
+// offset and the position along the line (the same for the whole wave) in
+// the SGPR offset, so walking the line costs no vector address arithmetic.
+// Every address the kernels form lies inside its buffer (positions are
+// clamped with scalar min/max); stores that must not happen use a VGPR
+// offset of kOOB, beyond every buffer, which the hardware range check drops.
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
-constexpr uint32_t kOOB = 0x80000000u;  // beyond every buffer used here (< 2 GiB each)
+typedef __attribute__((address_space(4))) const float cfloat;  // scalar-loaded 1-D metrics
+constexpr uint32_t kOOB = 0x80000000u;                          // all buffers here are < 2 GiB
 
 __device__ __forceinline__ rsrc_t make_rsrc(const void *base, size_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ float bload(rsrc_t r, uint32_t byte_off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0));
+__device__ __forceinline__ float bload(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
-__device__ __forceinline__ void bstore(rsrc_t r, uint32_t byte_off, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)byte_off, 0, 0);
+__device__ __forceinline__ void bstore(rsrc_t r, uint32_t voff, uint32_t soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
 }
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 constexpr int kPF = 4;  // prefetch distance (cells)
 
 template <int DIR, int LIM, bool COMP, int NST>
-__global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
+__device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   // the wave index is uniform across the wave: say so, so that the per-wave
   // buffer descriptors live in SGPRs
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nx = a.nx, ny = a.ny, nz = a.nz, n = a.n;
   // block -> (bin group, x tile, transverse line index); wave -> bin
   const int groups = a.ntrans2;
-  const int g = blockIdx.x % groups;
-  const int rest = blockIdx.x / groups;
+  const int g = bid % groups;
+  const int rest = bid / groups;
   const int tile = rest % a.ntrans;
   const int t2 = rest / a.ntrans + 1;  // k (y sweep) or j (z sweep)
   const int i = tile * 64 + lane + 1;
   const int b = g * a.nbin + wave;
   const bool wave_ok = (wave < a.nbin) && (b < a.nbins);
   const bool act = wave_ok && (i <= nx);
+  const int ia = (i <= nx) ? i : nx;  // idle lanes address a valid column
   const int bb = wave_ok ? b : 0;
 
   const size_t cbin = (size_t)(nx + 4) * (ny + 4) * (nz + 4);
@@ -154,53 +191,57 @@ __global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
   const rsrc_t rk = make_rsrc(a.kd, nkd * 4);
   const size_t nvs = (size_t)nx * ny * (nz + 3);
   const rsrc_t rvs = make_rsrc(DIR == 2 ? a.vs + nvs * bb : a.vel1, (DIR == 2 ? nvs : nvel) * 4);
-  const rsrc_t rh = make_rsrc(a.hdxp, (size_t)(n + 3) * 4);
-  const rsrc_t rib = make_rsrc(a.idxb, (size_t)(n + 3) * 4);
+  cfloat *hdxp = (cfloat *)a.hdxp;
+  cfloat *dtib = (cfloat *)a.dtib;
+  cfloat *hdtib = (cfloat *)a.hdtib;
 
-  // element offsets of position/face 0 of this lane's line, and strides
-  uint32_t cb0, cs, vb, vst, kb, kst;
+  // lane byte offsets of position -1 (c), face 0 (winds), cell -1 (K); byte strides
+  uint32_t vc, cs4, vv, vst4, vk, kst4;
   if constexpr (DIR == 1) {
-    cb0 = (uint32_t)((i + 1) + (nx + 4) * (1 + (size_t)(ny + 4) * (t2 + 1)));
-    cs = (uint32_t)(nx + 4);
-    vb = (uint32_t)((i - 1) + (size_t)nx * (ny + 3) * (t2 - 1));
-    vst = (uint32_t)nx;
-    kb = (uint32_t)((i - 1) + (size_t)nx * (ny + 4) * (t2 - 1));
-    kst = (uint32_t)nx;
+    vc = (uint32_t)(4 * ((size_t)(ia + 1) + (size_t)(nx + 4) * ((size_t)(ny + 4) * (t2 + 1))));
+    cs4 = (uint32_t)(4 * (nx + 4));
+    vv = (uint32_t)(4 * ((size_t)(ia - 1) + (size_t)nx * (ny + 3) * (t2 - 1)));
+    vst4 = (uint32_t)(4 * nx);
+    vk = (uint32_t)(4 * ((size_t)(ia - 1) + (size_t)nx * (ny + 4) * (t2 - 1)));
+    kst4 = (uint32_t)(4 * nx);
   } else {
-    cb0 = (uint32_t)((i + 1) + (nx + 4) * ((size_t)(t2 + 1) + (size_t)(ny + 4)));
-    cs = (uint32_t)((nx + 4) * (ny + 4));
-    vb = (uint32_t)((i - 1) + (size_t)nx * (t2 - 1));
-    vst = (uint32_t)(nx * ny);
-    kb = vb;
-    kst = vst;
+    vc = (uint32_t)(4 * ((size_t)(ia + 1) + (size_t)(nx + 4) * (t2 + 1)));
+    cs4 = (uint32_t)(4 * (size_t)(nx + 4) * (ny + 4));
+    vv = (uint32_t)(4 * ((size_t)(ia - 1) + (size_t)nx * (t2 - 1)));
+    vst4 = (uint32_t)(4 * (size_t)nx * ny);
+    vk = vv;
+    kst4 = vst4;
   }
-  const double area = act ? a.areaB[t2] * a.areaA[i] : 0.0;  // dX3_b(k) dX1_b(i) / dX2_b(j) dX1_b(i)
+  const uint32_t vstore = act ? vc : kOOB;
+  const double area = act ? a.areaB[t2] * a.areaA[ia] : 0.0;  // dX3_b(k) dX1_b(i) / dX2_b(j) dX1_b(i)
   const float s = a.stime, s1 = 1.0f - a.stime;
-  const float dt = a.dt;
 
+  auto cpos = [&](int P) -> uint32_t { return (uint32_t)clampi(P + 1, 0, n + 3) * cs4; };   // cell P of c
+  auto fpos = [&](int f) -> uint32_t { return (uint32_t)clampi(f, 0, n + 2) * vst4; };      // face f
+  auto kpos = [&](int P) -> uint32_t { return (uint32_t)clampi(P + 1, 0, n + 3) * kst4; };  // cell P of K
   auto face_v = [&](int f) -> float {  // wind (w_eff for z) at face f of this line
-    float v = s1 * bload(rv1, (vb + (uint32_t)f * vst) * 4u) + s * bload(rv2, (vb + (uint32_t)f * vst) * 4u);
-    if constexpr (DIR == 2) v -= bload(rvs, (vb + (uint32_t)f * vst) * 4u);
+    float v = s1 * bload(rv1, vv, fpos(f)) + s * bload(rv2, vv, fpos(f));
+    if constexpr (DIR == 2) v -= bload(rvs, vv, fpos(f));
     return v;
   };
 
   // free-flow ghosts from cells 1, n and faces 1, n+1 (Sources/mod_ADS.F90:1239-1324)
-  const float c1 = bload(rc, (cb0 + cs) * 4u);
-  const float cnn = bload(rc, (cb0 + (uint32_t)n * cs) * 4u);
+  const float c1 = bload(rc, vc, cpos(1));
+  const float cnn = bload(rc, vc, cpos(n));
   const float vfirst = face_v(1), vlast = face_v(n + 1);
   const float gL = (vfirst > 0.0f) ? (2.0f * 0.0f - c1) : c1;
   const float gR = (vlast < 0.0f) ? (2.0f * 0.0f - cnn) : cnn;
   {
-    const bool wg = act && a.write_ghosts;
-    bstore(rc, wg ? (cb0 - cs) * 4u : kOOB, gL);
-    bstore(rc, wg ? cb0 * 4u : kOOB, gL);
-    bstore(rc, wg ? (cb0 + (uint32_t)(n + 1) * cs) * 4u : kOOB, gR);
-    bstore(rc, wg ? (cb0 + (uint32_t)(n + 2) * cs) * 4u : kOOB, gR);
+    const uint32_t vg = a.write_ghosts ? vstore : kOOB;
+    bstore(rc, vg, cpos(-1), gL);
+    bstore(rc, vg, cpos(0), gL);
+    bstore(rc, vg, cpos(n + 1), gR);
+    bstore(rc, vg, cpos(n + 2), gR);
     if constexpr (COMP) {
-      bstore(rl, wg ? (cb0 - cs) * 4u : kOOB, 0.0f);
-      bstore(rl, wg ? cb0 * 4u : kOOB, 0.0f);
-      bstore(rl, wg ? (cb0 + (uint32_t)(n + 1) * cs) * 4u : kOOB, 0.0f);
-      bstore(rl, wg ? (cb0 + (uint32_t)(n + 2) * cs) * 4u : kOOB, 0.0f);
+      bstore(rl, vg, cpos(-1), 0.0f);
+      bstore(rl, vg, cpos(0), 0.0f);
+      bstore(rl, vg, cpos(n + 1), 0.0f);
+      bstore(rl, vg, cpos(n + 2), 0.0f);
     }
   }
 
@@ -208,133 +249,180 @@ __global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
   // raw face data (8) and face coefficients (8)
   float cn[16], lo[8];
   float rw1[8], rw2[8], rws[8], rkk[8];
-  float vf[8], kf[8], gh[8];
+  float up[8], um[8], kf[8], gh[8];
 #pragma unroll
   for (int q = 0; q < 16; ++q) cn[q] = 0.0f;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     rw1[q] = rw2[q] = rws[q] = rkk[q] = 0.0f;
-    vf[q] = kf[q] = gh[q] = lo[q] = 0.0f;
+    up[q] = um[q] = kf[q] = gh[q] = lo[q] = 0.0f;
   }
   // per-stage pipeline state: previous two inputs, last difference, last
-  // slope, last face flux
+  // slope, last face flux. Every stage starts as if it had consumed an
+  // unbounded run of west ghosts (value gL: zero difference and slope).
   float sa1[NST], sa2[NST], sd[NST], sm[NST], sG[NST];
 #pragma unroll
-  for (int st = 0; st < NST; ++st) sa1[st] = sa2[st] = sd[st] = sm[st] = sG[st] = gL;
+  for (int st = 0; st < NST; ++st) {
+    sa1[st] = sa2[st] = gL;
+    sd[st] = sm[st] = sG[st] = 0.0f;
+  }
+  // ring slots: cell p -> p & 15, face f -> f & 7. Preload what iterations
+  // t = -kPF..-1 would have fetched (cells -1..kPF-2, faces -2..kPF-3), so
+  // that the pipeline starts at t = 0.
+#pragma unroll
+  for (int p = -1; p <= kPF - 2; ++p) cn[p & 15] = bload(rc, vc, cpos(p));
+#pragma unroll
+  for (int f = -2; f <= kPF - 3; ++f) {
+    rw1[f & 7] = bload(rv1, vv, fpos(f));
+    rw2[f & 7] = bload(rv2, vv, fpos(f));
+    if constexpr (DIR == 2) rws[f & 7] = bload(rvs, vv, fpos(f));
+    rkk[f & 7] = bload(rk, vk, kpos(f));
+  }
 
-  float Fb = 0.0f, Pb = 0.0f, Fe = 0.0f, Pe = 0.0f, tl1 = 0.0f, trn = 0.0f;
+  float RFb = 0.0f, RFe = 0.0f, tl1 = 0.0f, trn = 0.0f;
   const int tend = n + 2 * NST + 1;  // iteration of the last final output (cell n)
 
-  for (int t0 = -16; t0 <= tend; t0 += 16) {
+  // One pipeline step at iteration t (u = t mod 16, a compile-time constant
+  // after unrolling). GEN: iteration that may touch a ghost, a boundary face
+  // or an out-of-range cell; otherwise every position is interior and the
+  // selects are compiled out.
+  auto step = [&](auto gen_tag, const int t, const int u) {
+    constexpr bool GEN = decltype(gen_tag)::value;
+    // ---- prefetch cell t+kPF-1 and face t+kPF-2
+    cn[(u + kPF - 1) & 15] = bload(rc, vc, cpos(t + kPF - 1));
+    if constexpr (COMP) lo[(u + kPF - 2 * NST - 1) & 7] = bload(rl, vc, cpos(t + kPF - 2 * NST - 1));
+    {
+      const int f = t + kPF - 2;
+      const uint32_t fo = fpos(f);
+      rw1[(u + kPF - 2) & 7] = bload(rv1, vv, fo);
+      rw2[(u + kPF - 2) & 7] = bload(rv2, vv, fo);
+      if constexpr (DIR == 2) rws[(u + kPF - 2) & 7] = bload(rvs, vv, fo);
+      rkk[(u + kPF - 2) & 7] = bload(rk, vk, kpos(f));
+    }
+    // ---- coefficients of face t-2 (first used by stage 1 in this iteration)
+    {
+      const int sl = (u - 2) & 7;
+      float v = s1 * rw1[sl] + s * rw2[sl];
+      if constexpr (DIR == 2) v -= rws[sl];
+      up[sl] = fmaxf(v, 0.0f);
+      um[sl] = fminf(v, 0.0f);
+      kf[sl] = (rkk[(u - 3) & 7] + rkk[sl]) * hdxp[clampi(t - 2, 0, n + 2)];
+    }
+    // ---- the pipeline: stage st consumes position t+1-2(st+1)
+    float v;
+    if constexpr (GEN) {
+      const int P = t - 1;
+      v = (P <= 0) ? gL : ((P > n) ? gR : cn[(u - 1) & 15]);
+    } else {
+      v = cn[(u - 1) & 15];
+    }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int t = t0 + u;
-      // ---- prefetch cell t+kPF-1 and face t+kPF-2 (edge/overshoot reads are
-      // harmless: in-buffer cells of other lines or 0 from the range check)
-      {
-        const int P = t + kPF - 1;
-        cn[(u + kPF - 1) & 15] = bload(rc, (cb0 + (uint32_t)P * cs) * 4u);
-        if constexpr (COMP) {
-          const int Q = t + kPF - 2 * NST - 1;  // cell finalised kPF iterations from now
-          lo[(u + kPF - 2 * NST - 1) & 7] = bload(rl, (cb0 + (uint32_t)Q * cs) * 4u);
+    for (int st = 0; st < NST; ++st) {
+      const int f = t - 2 * (st + 1);  // face completed by this stage
+      const int fs = (u - 2 * (st + 1)) & 7;
+      const float d = v - sa1[st];
+      const float m = limiter_slope<LIM>(sd[st], d);
+      const float cL = fmaf(0.5f, sm[st], sa2[st]);
+      const float cR = fmaf(-0.5f, m, sa1[st]);
+      const float F = fmaf(up[fs], cL, um[fs] * cR);  // KT flux with a = |u| (upwind)
+      const float Pd = kf[fs] * sd[st];
+      const float G = F - Pd;
+      float wgt;
+      if constexpr (NST == 1)
+        wgt = 1.0f;
+      else
+        wgt = (st == 0 || st == 3) ? (1.0f / 6.0f) : (1.0f / 3.0f);
+      gh[fs] = (st == 0) ? wgt * G : fmaf(wgt, G, gh[fs]);
+      if constexpr (GEN) {
+        if (st == NST - 1) {  // reference end-face fluxes F + P of the last stage
+          RFb = (f == 1) ? F + Pd : RFb;
+          RFe = (f == n + 1) ? F + Pd : RFe;
         }
-        const int f = t + kPF - 2;
-        const uint32_t fo = (vb + (uint32_t)f * vst) * 4u;
-        rw1[(u + kPF - 2) & 7] = bload(rv1, fo);
-        rw2[(u + kPF - 2) & 7] = bload(rv2, fo);
-        if constexpr (DIR == 2) rws[(u + kPF - 2) & 7] = bload(rvs, fo);
-        rkk[(u + kPF - 2) & 7] = bload(rk, (kb + (uint32_t)(f + 1) * kst) * 4u);
       }
-      // ---- coefficients of face t-2 (first used by stage 1 in this iteration)
-      {
-        const int f = t - 2;
-        const int sl = (u - 2) & 7;
-        float v = s1 * rw1[sl] + s * rw2[sl];
-        if constexpr (DIR == 2) v -= rws[sl];
-        vf[sl] = v;
-        kf[sl] = (rkk[(u - 3) & 7] + rkk[sl]) * bload(rh, (uint32_t)f * 4u);
-      }
-      // ---- the pipeline: stage st consumes position t+1-2(st+1)
-      float v;
-      {
-        const int P = t - 1;
-        v = (P <= 0) ? gL : ((P > n) ? gR : cn[(u - 1) & 15]);
-      }
-#pragma unroll
-      for (int st = 0; st < NST; ++st) {
-        const int f = t - 2 * (st + 1);  // face completed by this stage
-        const int fs = (u - 2 * (st + 1)) & 7;
-        const float d = v - sa1[st];
-        const float m = limiter_slope<LIM>(sd[st], d);
-        const float cL = fmaf(0.5f, sm[st], sa2[st]);
-        const float cR = fmaf(-0.5f, m, sa1[st]);
-        const float uf = vf[fs];
-        const float F = uf * ((uf > 0.0f) ? cL : cR);  // upwind = KT flux with a = |u|
-        const float Pd = kf[fs] * sd[st];
-        const float G = F - Pd;
-        float wgt;
-        if constexpr (NST == 1)
-          wgt = 1.0f;
-        else
-          wgt = (st == 0 || st == 3) ? (1.0f / 6.0f) : (1.0f / 3.0f);
-        gh[fs] = (st == 0) ? wgt * G : fmaf(wgt, G, gh[fs]);
-        if (st == NST - 1) {
-          Fb = (f == 1) ? F : Fb;
-          Pb = (f == 1) ? Pd : Pb;
-          Fe = (f == n + 1) ? F : Fe;
-          Pe = (f == n + 1) ? Pd : Pe;
-        }
-        const int q = f - 1;  // output cell of this stage
-        const float Gp = sG[st];
-        sa2[st] = sa1[st];
-        sa1[st] = v;
-        sd[st] = d;
-        sm[st] = m;
-        sG[st] = G;
-        if (st < NST - 1) {
-          const float h = (st == 2) ? dt : 0.5f * dt;
-          const float yn = fmaf(h * bload(rib, (uint32_t)q * 4u), Gp - G, cn[(u - 2 * (st + 1) - 1) & 15]);
+      const int q = f - 1;  // output cell of this stage
+      const float Gp = sG[st];
+      sa2[st] = sa1[st];
+      sa1[st] = v;
+      sd[st] = d;
+      sm[st] = m;
+      sG[st] = G;
+      if (st < NST - 1) {
+        const float hib = (st == 2) ? dtib[clampi(q, 0, n + 2)] : hdtib[clampi(q, 0, n + 2)];
+        const float yn = fmaf(hib, Gp - G, cn[(u - 2 * (st + 1) - 1) & 15]);
+        if constexpr (GEN)
           v = (q <= 0) ? gL : ((q > n) ? gR : yn);  // next stage input (position q)
-        }
+        else
+          v = yn;
       }
-      // ---- final update of cell q = t - 2 NST - 1
-      {
-        const int q = t - 2 * NST - 1;
-        const bool qok = act && q >= 1 && q <= n;
-        const uint32_t off = qok ? (cb0 + (uint32_t)q * cs) * 4u : kOOB;
-        const int cl = (u - 2 * NST - 1) & 15;
-        const float ib = bload(rib, (uint32_t)q * 4u);
-        const float gq = gh[(u - 2 * NST - 1) & 7], gq1 = gh[(u - 2 * NST) & 7];
-        const float tl = (dt * gq) * ib;
-        const float tr = (dt * gq1) * ib;
-        tl1 = (q == 1) ? tl : tl1;
-        trn = (q == n) ? tr : trn;
-        if constexpr (COMP) {
-          float hv = cn[cl], lv = lo[(u - 2 * NST - 1) & 7];
-          comp_add(hv, lv, tl);
-          comp_add(hv, lv, -tr);
-          bstore(rc, off, hv);
-          bstore(rl, off, lv);
-        } else {
-          bstore(rc, off, fmaf(dt * (gq - gq1), ib, cn[cl]));
-        }
+    }
+    // ---- final update of cell q = t - 2 NST - 1
+    {
+      const int q = t - 2 * NST - 1;
+      const int cl = (u - 2 * NST - 1) & 15;
+      const float dtq = dtib[clampi(q, 0, n + 2)];
+      const float gq = gh[(u - 2 * NST - 1) & 7], gq1 = gh[(u - 2 * NST) & 7];
+      uint32_t vo = vstore;
+      if constexpr (GEN) {
+        vo = (q >= 1 && q <= n) ? vstore : kOOB;
+        tl1 = (q == 1) ? dtq * gq : tl1;
+        trn = (q == n) ? dtq * gq1 : trn;
+      }
+      if constexpr (COMP) {
+        float hv = cn[cl], lv = lo[(u - 2 * NST - 1) & 7];
+        comp_add(hv, lv, dtq * gq);
+        comp_add(hv, lv, -(dtq * gq1));
+        bstore(rc, vo, cpos(q), hv);
+        bstore(rl, vo, cpos(q), lv);
+      } else {
+        bstore(rc, vo, cpos(q), fmaf(dtq, gq - gq1, cn[cl]));
+      }
+    }
+  };
+
+  // Blocks of 16 iterations. Interior blocks (every t in [2 NST + 3, n + 1]:
+  // no ghost input, no boundary face, no capture, every output in range) run
+  // without selects; the others stop after iteration tend.
+  for (int t0 = 0; t0 <= tend; t0 += 16) {
+    if (t0 >= 2 * NST + 3 && t0 + 15 <= n + 1) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) step(std::false_type{}, t0 + u, u);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        if (t0 + u > tend) break;
+        step(std::true_type{}, t0 + u, u);
       }
     }
   }
 
   double rb = 0.0, re = 0.0, lb = 0.0, le = 0.0;
   if (act) {
-    const float frb = Fb + Pb, fre = Fe + Pe;
-    rb = (double)frb * area * (double)dt;
-    re = (double)fre * area * (double)dt;
+    const float dt = a.dt;
+    rb = (double)RFb * area * (double)dt;
+    re = (double)RFe * area * (double)dt;
     lb = -(double)tl1 * a.dxb64[1] * area;
     le = (double)trn * a.dxb64[n] * area;
     if constexpr (DIR == 2) {
       const size_t ai = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(t2 + 1) + (size_t)(ny + 4) * bb);
-      a.acum[ai] = a.acum[ai] + fabs((double)frb) * (double)dt;
+      a.acum[ai] = a.acum[ai] + fabs((double)RFb) * (double)dt;
     }
   }
-  block_partials(rb, re, lb, le, a.partials, blockIdx.x);
+  block_partials(rb, re, lb, le, a.partials, bid);
+}
+
+template <int DIR, int LIM, bool COMP, int NST>
+__global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
+  stream_body<DIR, LIM, COMP, NST>(a, blockIdx.x);
+}
+
+// All members of an ensemble in one launch: the workgroup finds its member
+// in the prefix table of workgroup counts, reads that member's arguments
+// (scalar loads) and runs the same body on its member-local block index.
+template <int DIR, int LIM, bool COMP, int NST>
+__global__ void __launch_bounds__(512) k_stream_batch(SweepBatch b) {
+  const int m = batch_member(b, blockIdx.x);
+  const SweepArgs a = member_args(b, m);
+  stream_body<DIR, LIM, COMP, NST>(a, blockIdx.x - ((cint *)b.bstart)[m]);
 }
 
 // =============================================================================
@@ -342,14 +430,14 @@ __global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
 // =============================================================================
 
 template <int C, int LIM, bool COMP, int NST>
-__global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a) {
+__device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, const int by) {
   static_assert(C >= 2, "segments need two edge cells");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int S = a.S, n = a.n;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int lpw = 64 / S;  // lines per wave
   const int lslot = lane / S, seg = lane % S;
-  const int line = (blockIdx.x * (blockDim.x >> 6) + wave) * lpw + lslot;
+  const int line = (bx * (blockDim.x >> 6) + wave) * lpw + lslot;
   const bool act = (lslot < lpw) && (line < ny * nz);
   const int j = act ? line % ny + 1 : 1;
   const int k = act ? line / ny + 1 : 1;
@@ -400,7 +488,7 @@ __global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a) {
   }
 
   double acc_rb = 0.0, acc_re = 0.0, acc_lb = 0.0, acc_le = 0.0;
-  const int bfirst = a.bin0 + blockIdx.y * a.nbin;
+  const int bfirst = a.bin0 + by * a.nbin;
   const int blast = min(bfirst + a.nbin, a.nbins);
 
   // prefetch the first bin
@@ -565,7 +653,20 @@ __global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a) {
       }
     }
   }
-  block_partials(acc_rb, acc_re, acc_lb, acc_le, a.partials, (size_t)blockIdx.y * gridDim.x + blockIdx.x);
+  block_partials(acc_rb, acc_re, acc_lb, acc_le, a.partials, (size_t)by * a.gx + bx);
+}
+
+template <int C, int LIM, bool COMP, int NST>
+__global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a) {
+  sweep_x_body<C, LIM, COMP, NST>(a, blockIdx.x, blockIdx.y);
+}
+
+template <int C, int LIM, bool COMP, int NST>
+__global__ void __launch_bounds__(256) k_sweep_x_batch(SweepBatch b) {
+  const int m = batch_member(b, blockIdx.x);
+  const SweepArgs a = member_args(b, m);
+  const int lb = blockIdx.x - ((cint *)b.bstart)[m];
+  sweep_x_body<C, LIM, COMP, NST>(a, lb % a.gx, lb / a.gx);
 }
 
 }  // namespace f3d

@@ -22,6 +22,32 @@ static hipError_t launch_x_c(int limiter, bool comp, int nst, const SweepArgs &a
   return hipErrorInvalidValue;
 }
 
+template <int C>
+static hipError_t launch_xb_c(int limiter, bool comp, int nst, const SweepBatch &b, int nblocks, dim3 block,
+                              hipStream_t st) {
+#define F3D_L(L, CP, N)                                                                     \
+  if (limiter == L && comp == CP && nst == N) {                                             \
+    hipLaunchKernelGGL((k_sweep_x_batch<C, L, CP, N>), dim3(nblocks), block, 0, st, b);     \
+    return hipGetLastError();                                                               \
+  }
+  F3D_L(1, false, 4) F3D_L(2, false, 4) F3D_L(3, false, 4)
+  F3D_L(1, true, 4) F3D_L(2, true, 4) F3D_L(3, true, 4)
+  F3D_L(1, false, 1) F3D_L(2, false, 1) F3D_L(3, false, 1)
+  F3D_L(1, true, 1) F3D_L(2, true, 1) F3D_L(3, true, 1)
+#undef F3D_L
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_sweep_x_batch(int cpt, int limiter, bool comp, int nst, const SweepBatch &b, int nblocks,
+                                dim3 block, hipStream_t st) {
+  (void)hipGetLastError();
+#define F3D_CASE(CPT) \
+  if (cpt == CPT) return launch_xb_c<CPT>(limiter, comp, nst, b, nblocks, block, st);
+  F3D_CPT_LIST(F3D_CASE)
+#undef F3D_CASE
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_sweep_x(int cpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,
                           hipStream_t st) {
   (void)hipGetLastError();
