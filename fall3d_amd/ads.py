@@ -26,7 +26,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libfall3d_ads_hip.so")
+LIB_PATH = os.environ.get("FALL3D_ADS_LIB") or os.path.join(_HERE, "lib", "libfall3d_ads_hip.so")
 
 MINMOD, SUPERBEE, OSPRE = 1, 2, 3
 EULER, RK = 1, 2
@@ -49,6 +49,15 @@ EXPORTS = (
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
     "fall3d_ads_set_graph",
+    "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
+    "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
+    "fall3d_ads_ensemble_set_graph", "fall3d_ads_ensemble_synchronize", "fall3d_ads_ensemble_set_tiling",
+    "fall3d_ads_ensemble_get_tiling", "fall3d_ads_ensemble_autotune",
+    "fall3d_mlp_create", "fall3d_mlp_release", "fall3d_mlp_last_error", "fall3d_mlp_nparam",
+    "fall3d_mlp_set_params", "fall3d_mlp_get_params", "fall3d_mlp_forward", "fall3d_mlp_gradient",
+    "fall3d_mlp_data_append", "fall3d_mlp_data_count", "fall3d_mlp_data_get", "fall3d_mlp_data_clear",
+    "fall3d_mlp_train", "fall3d_ads_set_flux_output", "fall3d_ads_get_face_flux", "fall3d_ads_set_learned",
+    "fall3d_ads_coarse_grain", "fall3d_ads_learned_collect",
 )
 
 
@@ -100,6 +109,37 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_get_kernel_time": ([_vp, _int, _dp, ctypes.POINTER(ctypes.c_longlong)], _int),
         "fall3d_ads_sweep_bytes": ([_vp, _int, _dp], _int),
         "fall3d_ads_set_graph": ([_vp, _int], _int),
+        "fall3d_ads_ensemble_create": ([ctypes.POINTER(_vp), _int, ctypes.POINTER(_int), _int, _int, _int, _int,
+                                        _int], _int),
+        "fall3d_ads_ensemble_release": ([_vp], _int),
+        "fall3d_ads_ensemble_last_error": ([_vp], ctypes.c_char_p),
+        "fall3d_ads_ensemble_member": ([_vp, _int, ctypes.POINTER(_vp)], _int),
+        "fall3d_ads_ensemble_step": ([_vp, _dp, _dp, _int], _int),
+        "fall3d_ads_ensemble_run": ([_vp, _dp, _dp, _int, _int], _int),
+        "fall3d_ads_ensemble_set_graph": ([_vp, _int], _int),
+        "fall3d_ads_ensemble_synchronize": ([_vp], _int),
+        "fall3d_ads_ensemble_set_tiling": ([_vp, _int, _int, _int], _int),
+        "fall3d_ads_ensemble_get_tiling": ([_vp, ctypes.POINTER(_int)], _int),
+        "fall3d_ads_ensemble_autotune": ([_vp, _dp, _dp, _int, _dp], _int),
+        "fall3d_mlp_create": ([ctypes.POINTER(_vp), ctypes.c_long], _int),
+        "fall3d_mlp_release": ([_vp], _int),
+        "fall3d_mlp_last_error": ([_vp], ctypes.c_char_p),
+        "fall3d_mlp_nparam": ([], _int),
+        "fall3d_mlp_set_params": ([_vp, _vp], _int),
+        "fall3d_mlp_get_params": ([_vp, _vp], _int),
+        "fall3d_mlp_forward": ([_vp, _vp, ctypes.c_long, _vp], _int),
+        "fall3d_mlp_gradient": ([_vp, _vp, _vp, ctypes.c_long, _vp, _dp], _int),
+        "fall3d_mlp_data_append": ([_vp, _vp, _vp, ctypes.c_long], _int),
+        "fall3d_mlp_data_count": ([_vp, ctypes.POINTER(ctypes.c_long)], _int),
+        "fall3d_mlp_data_get": ([_vp, ctypes.c_long, ctypes.c_long, _vp, _vp], _int),
+        "fall3d_mlp_data_clear": ([_vp], _int),
+        "fall3d_mlp_train": ([_vp, _int, _int, ctypes.c_float, ctypes.c_ulonglong, _vp], _int),
+        "fall3d_ads_set_flux_output": ([_vp, _int], _int),
+        "fall3d_ads_get_face_flux": ([_vp, _int, _vp], _int),
+        "fall3d_ads_set_learned": ([_vp, _vp, _dbl], _int),
+        "fall3d_ads_coarse_grain": ([_vp, _vp], _int),
+        "fall3d_ads_learned_collect": ([_vp, _vp, _vp, _dbl, _dbl, _int, _dbl, ctypes.POINTER(ctypes.c_long)],
+                                       _int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -117,8 +157,9 @@ class ADS:
     """One ADS instance (a context of libfall3d_ads_hip.so) for an nx*ny*nz*nbins domain."""
 
     def __init__(self, nx, ny, nz, nbins, limiter="minmod", time_marching="rk", real_kind=8, state="fp32",
-                 grid=None):
+                 grid=None, _borrow=None):
         L = load_library()
+        self._borrowed = _borrow is not None
         self.nx, self.ny, self.nz, self.nb = int(nx), int(ny), int(nz), int(nbins)
         self.kind = int(real_kind)
         self.dtype = _kind_dtype(self.kind)
@@ -126,6 +167,10 @@ class ADS:
         tm = TIME_MARCHING[time_marching] if isinstance(time_marching, str) else int(time_marching)
         st = STATES[state] if isinstance(state, str) else int(state)
         self.limiter, self.time_marching, self.state = lim, tm, st
+        self._keep = {}
+        if _borrow is not None:  # a member context owned by an Ensemble
+            self._h = _borrow
+            return
         h = _vp()
         rc = L.fall3d_ads_initialize(ctypes.byref(h), self.nx, self.ny, self.nz, self.nb, lim, tm, self.kind, st)
         self._h = h
@@ -134,7 +179,6 @@ class ADS:
             L.fall3d_ads_release(h)
             self._h = None
             raise Fall3dAdsError(f"fall3d_ads_initialize failed ({rc}): {msg}")
-        self._keep = {}
         if grid is not None:
             self.set_grid(grid)
 
@@ -267,7 +311,8 @@ class ADS:
 
     def release(self):
         if getattr(self, "_h", None):
-            load_library().fall3d_ads_release(self._h)
+            if not self._borrowed:
+                load_library().fall3d_ads_release(self._h)
             self._h = None
 
     def __del__(self):
@@ -304,9 +349,10 @@ class ADS:
         self._chk(load_library().fall3d_ads_set_source(self._h, self._p(s)), "set_source")
 
     def set_state(self, c, acum=None):
-        c = self._arr(c, self.shape_c, "c")
+        """Upload c (None: zero field) and acum (None: zero)."""
+        c = self._arr(c, self.shape_c, "c") if c is not None else None
         acum = self._arr(acum, self.shape_acum, "acum") if acum is not None else None
-        self._chk(load_library().fall3d_ads_set_state(self._h, c.ctypes.data, self._p(acum)), "set_state")
+        self._chk(load_library().fall3d_ads_set_state(self._h, self._p(c), self._p(acum)), "set_state")
 
     def get_state(self):
         c = np.zeros(self.shape_c, dtype=self.dtype, order="F")
@@ -367,6 +413,24 @@ class ADS:
                   "get_kernel_time")
         return ms.value, n.value
 
+    # -- learned flux correction ---------------------------------------------------
+    def set_flux_output(self, on=True):
+        """Accumulate the x/y face mass fluxes of later sweeps (zeroed by this call)."""
+        self._chk(load_library().fall3d_ads_set_flux_output(self._h, int(bool(on))), "set_flux_output")
+
+    def face_flux(self, axis):
+        """Accumulated face fluxes: axis 0 -> (nx+3, ny, nz, nb), axis 1 -> (nx, ny+3, nz, nb) (x-fastest)."""
+        shape = (self.nx + 3, self.ny, self.nz, self.nb) if axis == 0 else (self.nx, self.ny + 3, self.nz, self.nb)
+        out = np.zeros(shape, dtype=np.float32, order="F")
+        self._chk(load_library().fall3d_ads_get_face_flux(self._h, int(axis), out.ctypes.data), "get_face_flux")
+        return out
+
+    def set_learned(self, net=None, smin=0.0):
+        """Correct the x/y face fluxes of later steps with net (None: plain KT)."""
+        self._keep["learned"] = net
+        self._chk(load_library().fall3d_ads_set_learned(self._h, net._h if net is not None else None, float(smin)),
+                  "set_learned")
+
     def sweep_bytes(self, axis):
         b = _dbl()
         self._chk(load_library().fall3d_ads_sweep_bytes(self._h, int(axis), ctypes.byref(b)), "sweep_bytes")
@@ -375,6 +439,218 @@ class ADS:
 
 # ---- module-level singleton, as the reference's private CB (mod_ADS.F90:41) ----
 _default: ADS | None = None
+
+
+class Ensemble:
+    """Ensemble members of different sizes advanced together (one batched
+    launch per sweep, whole runs replayed from one hipGraph). members[m] is an
+    ADS object for member m (fill it with set_problem/set_state, read it with
+    get_state/budget); it is owned by the ensemble."""
+
+    def __init__(self, dims, nbins, limiter="minmod", time_marching="rk", real_kind=8, state="fp32"):
+        L = load_library()
+        dims = [tuple(int(v) for v in d) for d in dims]
+        self.nmem, self.nb = len(dims), int(nbins)
+        lim = LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
+        tm = TIME_MARCHING[time_marching] if isinstance(time_marching, str) else int(time_marching)
+        st = STATES[state] if isinstance(state, str) else int(state)
+        flat = (_int * (3 * self.nmem))(*[v for d in dims for v in d])
+        h = _vp()
+        rc = L.fall3d_ads_ensemble_create(ctypes.byref(h), self.nmem, flat, self.nb, lim, tm, int(real_kind), st)
+        self._h = h
+        if rc != 0:
+            msg = L.fall3d_ads_ensemble_last_error(h).decode()
+            L.fall3d_ads_ensemble_release(h)
+            self._h = None
+            raise Fall3dAdsError(f"fall3d_ads_ensemble_create failed ({rc}): {msg}")
+        self.members = []
+        for m, d in enumerate(dims):
+            mh = _vp()
+            self._chk(L.fall3d_ads_ensemble_member(h, m, ctypes.byref(mh)), "ensemble_member")
+            self.members.append(ADS(*d, self.nb, lim, tm, real_kind, st, _borrow=mh))
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = load_library().fall3d_ads_ensemble_last_error(self._h).decode()
+            raise Fall3dAdsError(f"{what} failed ({rc}): {msg}")
+
+    def _vec(self, v, default=0.0):
+        if v is None:
+            v = [default] * self.nmem
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.nmem,)))
+        return a, a.ctypes.data_as(_dp)
+
+    def step(self, dt, iiter, stime=None):
+        (a, pa), (b, pb) = self._vec(dt), self._vec(stime)
+        self._chk(load_library().fall3d_ads_ensemble_step(self._h, pa, pb, int(iiter)), "ensemble_step")
+
+    def run(self, dt, nsteps, iiter0=0, stime=None):
+        (a, pa), (b, pb) = self._vec(dt), self._vec(stime)
+        self._chk(load_library().fall3d_ads_ensemble_run(self._h, pa, pb, int(iiter0), int(nsteps)), "ensemble_run")
+
+    def synchronize(self):
+        self._chk(load_library().fall3d_ads_ensemble_synchronize(self._h), "ensemble_synchronize")
+
+    def set_graph(self, on=True):
+        self._chk(load_library().fall3d_ads_ensemble_set_graph(self._h, int(bool(on))), "ensemble_set_graph")
+
+    def set_tiling(self, x_cells_per_lane=0, y_bins_per_group=0, z_bins_per_group=0):
+        self._chk(load_library().fall3d_ads_ensemble_set_tiling(self._h, int(x_cells_per_lane), int(y_bins_per_group),
+                                                                int(z_bins_per_group)), "ensemble_set_tiling")
+
+    def tiling(self):
+        out = (_int * 3)()
+        self._chk(load_library().fall3d_ads_ensemble_get_tiling(self._h, out), "ensemble_get_tiling")
+        return tuple(out)
+
+    def autotune(self, dt, stime=None, reps=3):
+        """Time every candidate tile shape per sweep, keep the fastest; returns
+        (tiling, per-sweep milliseconds)."""
+        (a, pa), (b, pb) = self._vec(dt), self._vec(stime)
+        ms = (ctypes.c_double * 3)()
+        self._chk(load_library().fall3d_ads_ensemble_autotune(self._h, pa, pb, int(reps), ms), "ensemble_autotune")
+        return self.tiling(), tuple(ms)
+
+    def release(self):
+        if getattr(self, "_h", None):
+            for mobj in self.members:
+                mobj._h = None
+            load_library().fall3d_ads_ensemble_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Network:
+    """The per-face flux-correction network (BF16 MFMA kernels, FP32 Adam) and
+    its training rows on the device."""
+
+    def __init__(self, capacity=0, params=None):
+        L = load_library()
+        h = _vp()
+        rc = L.fall3d_mlp_create(ctypes.byref(h), int(capacity))
+        self._h = h
+        if rc != 0:
+            msg = L.fall3d_mlp_last_error(h).decode()
+            L.fall3d_mlp_release(h)
+            self._h = None
+            raise Fall3dAdsError(f"fall3d_mlp_create failed ({rc}): {msg}")
+        self.nparam = L.fall3d_mlp_nparam()
+        if params is not None:
+            self.set_params(params)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = load_library().fall3d_mlp_last_error(self._h).decode()
+            raise Fall3dAdsError(f"{what} failed ({rc}): {msg}")
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        if p.size != self.nparam:
+            raise ValueError(f"expected {self.nparam} parameters, got {p.size}")
+        self._chk(load_library().fall3d_mlp_set_params(self._h, p.ctypes.data), "mlp_set_params")
+
+    def params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        self._chk(load_library().fall3d_mlp_get_params(self._h, p.ctypes.data), "mlp_get_params")
+        return p
+
+    @staticmethod
+    def _rows(x):
+        x = np.ascontiguousarray(x, dtype=np.uint16)
+        if x.ndim != 2 or x.shape[1] != 32:
+            raise ValueError("feature rows must be (n, 32) bfloat16 bit patterns")
+        return x
+
+    def forward(self, x):
+        x = self._rows(x)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        self._chk(load_library().fall3d_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data), "mlp_forward")
+        return y
+
+    def gradient(self, x, t):
+        x = self._rows(x)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        sse = ctypes.c_double()
+        self._chk(load_library().fall3d_mlp_gradient(self._h, x.ctypes.data, t.ctypes.data, x.shape[0], g.ctypes.data,
+                                                     ctypes.byref(sse)), "mlp_gradient")
+        return g, sse.value
+
+    def append(self, x, t):
+        x = self._rows(x)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._chk(load_library().fall3d_mlp_data_append(self._h, x.ctypes.data, t.ctypes.data, x.shape[0]),
+                  "mlp_data_append")
+
+    def count(self):
+        n = ctypes.c_long()
+        self._chk(load_library().fall3d_mlp_data_count(self._h, ctypes.byref(n)), "mlp_data_count")
+        return n.value
+
+    def rows(self, first=0, n=None):
+        n = self.count() - first if n is None else n
+        x = np.zeros((n, 32), dtype=np.uint16)
+        t = np.zeros(n, dtype=np.float32)
+        self._chk(load_library().fall3d_mlp_data_get(self._h, int(first), int(n), x.ctypes.data, t.ctypes.data),
+                  "mlp_data_get")
+        return x, t
+
+    def clear(self):
+        self._chk(load_library().fall3d_mlp_data_clear(self._h), "mlp_data_clear")
+
+    def train(self, steps, batch=65536, lr=1e-3, seed=0):
+        """steps Adam updates on random batches of the stored rows; returns the
+        mean squared error of each batch."""
+        loss = np.zeros(max(int(steps), 1), dtype=np.float32)
+        self._chk(load_library().fall3d_mlp_train(self._h, int(steps), int(batch), float(lr), int(seed),
+                                                  loss.ctypes.data), "mlp_train")
+        return loss[:steps]
+
+    def release(self):
+        if getattr(self, "_h", None):
+            load_library().fall3d_mlp_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def coarse_grain(fine, coarse):
+    """Interior of coarse := 2x2 horizontal means of fine."""
+    rc = load_library().fall3d_ads_coarse_grain(fine._h, coarse._h)
+    coarse._chk(rc, "coarse_grain")
+
+
+def learned_collect(fine, coarse, net, dt, stime=0.0, iiter=0, smin=0.0):
+    """One training sample of the whole coarse grid (see fall3d_ads_learned_collect);
+    returns the number of rows added."""
+    n = ctypes.c_long()
+    rc = load_library().fall3d_ads_learned_collect(fine._h, coarse._h, net._h, float(dt), float(stime), int(iiter),
+                                                  float(smin), ctypes.byref(n))
+    if rc != 0:
+        net._chk(rc, "learned_collect")
+    return n.value
+
+
+def bf16_bits(a):
+    """float array -> bfloat16 bit patterns (round to nearest even), uint16."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def bf16_value(bits):
+    """bfloat16 bit patterns -> float32 values."""
+    return (np.ascontiguousarray(bits, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
 def ADS_initialize(limitertype, time_marching, nx, ny, nz, nbins, grid, real_kind=8, state="fp32"):

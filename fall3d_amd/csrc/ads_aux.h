@@ -14,6 +14,26 @@ struct ReduceJob {
   int count;
 };
 
+// One reduction of an ensemble step: dst += sum_b part[b*4 + slot].
+struct BatchJob {
+  const double *part;
+  double *dst;
+  int nblocks;
+  int slot;
+};
+
+// Sparse source of one ensemble member (see source_inject); the injected
+// mass of workgroup b goes to part[b*4].
+struct SourceMember {
+  float *chi, *clo;
+  const size_t *idx;
+  const double *rate_vol, *meas;
+  double *part;
+  double dt;
+  int nent;
+  int pad;
+};
+
 struct CflArgs {
   int nx, ny, nz, nb, criterion;
   double stime, cfl;
@@ -43,6 +63,8 @@ hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, h
 hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double *rate_vol, const double *meas,
                          int nent, double dt, double *partials, int *nblocks, hipStream_t st);
 hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st);
+hipError_t finalize_batch(const BatchJob *jobs, int njobs, hipStream_t st);
+hipError_t source_inject_batch(const SourceMember *tab, const int *bstart, int nmem, int nblocks, hipStream_t st);
 hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *out, hipStream_t st);
 hipError_t mass(const float *chi, const float *clo, int nx, int ny, int nz, int nb, const double *d1,
                 const double *d2, const double *d3, int off, double *part, int part_cap, double *out,

@@ -128,18 +128,20 @@ hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, h
 }
 
 // dt/dxb(p) and 0.5 dt/dxb(p) in FP32 for the RK stage updates of one axis
-__global__ void k_dt_over_dxb(const double *__restrict__ dxb, int n3, double dt, float *__restrict__ dtib,
-                              float *__restrict__ hdtib) {
+__global__ void k_dt_over_dxb(const double *__restrict__ dxb, int n3, double dt, double wfinal,
+                              float *__restrict__ tab) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n3) {
-    dtib[p] = (float)(dt / dxb[p]);
-    hdtib[p] = (float)(0.5 * dt / dxb[p]);
+    tab[p] = (float)(dt / dxb[p]);
+    tab[n3 + p] = (float)(0.5 * dt / dxb[p]);
+    tab[2 * n3 + p] = (float)(wfinal * dt / dxb[p]);
   }
 }
 
-hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, float *dtib, float *hdtib, hipStream_t st) {
+hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, hipStream_t st) {
   (void)hipGetLastError();
-  hipLaunchKernelGGL(k_dt_over_dxb, dim3((n3 + 255) / 256), dim3(256), 0, st, dxb64, n3, dt, dtib, hdtib);
+  const double wfinal = (tm == 1) ? 1.0 : 1.0 / 6.0;
+  hipLaunchKernelGGL(k_dt_over_dxb, dim3((n3 + 255) / 256), dim3(256), 0, st, dxb64, n3, dt, wfinal, tab);
   return hipGetLastError();
 }
 
@@ -219,6 +221,62 @@ __global__ void k_source(float *__restrict__ chi, float *__restrict__ clo, const
   }
 }
 
+// Ensemble form: the workgroup finds its member in the prefix table of
+// workgroup counts and injects that member's entries.
+__global__ void k_source_batch(const SourceMember *__restrict__ tab, const int *__restrict__ bstart, int nmem) {
+  int lo = 0, hi = nmem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bstart[mid] <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SourceMember sm = tab[lo];
+  const int lb = blockIdx.x - bstart[lo];
+  const int e = lb * blockDim.x + threadIdx.x;
+  double m = 0.0;
+  if (e < sm.nent) {
+    const double inc = sm.dt * sm.rate_vol[e];
+    const size_t i = sm.idx[e];
+    if (sm.clo) {
+      const float ih = (float)inc;
+      const float il = (float)(inc - (double)ih);
+      float h = sm.chi[i], l = sm.clo[i];
+      comp_add2(h, l, ih);
+      comp_add2(h, l, il);
+      sm.chi[i] = h;
+      sm.clo[i] = l;
+      m = inc * sm.meas[e];
+    } else {
+      const float old = sm.chi[i];
+      const float nv = old + (float)inc;
+      sm.chi[i] = nv;
+      m = ((double)nv - (double)old) * sm.meas[e];
+    }
+  }
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  m = wsum(m);
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+      const int nw = (blockDim.x + 63) >> 6;
+      for (int w = 0; w < nw; ++w) s += red[w];
+    }
+    sm.part[(size_t)lb * 4 + threadIdx.x] = s;
+  }
+}
+
+hipError_t source_inject_batch(const SourceMember *tab, const int *bstart, int nmem, int nblocks, hipStream_t st) {
+  (void)hipGetLastError();
+  if (nblocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_source_batch, dim3(nblocks), dim3(256), 0, st, tab, bstart, nmem);
+  return hipGetLastError();
+}
+
 hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double *rate_vol, const double *meas,
                          int nent, double dt, double *partials, int *nblocks, hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
@@ -235,16 +293,46 @@ hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double
 
 // One wave per (array, slot): sums partials[b*4+slot] over b in a fixed
 // lane-strided order, then a fixed shuffle tree; adds into acc[dst].
+// Sum of p[b*4 + slot] over b < nb by one wave, in a fixed order: lane l
+// keeps 8 running sums over b = l + 64 (8 k + r) (independent loads in
+// flight), combines them r = 0..7, then the shuffle tree.
+__device__ __forceinline__ double wave_slot_sum(const double *__restrict__ p, int nb, int slot) {
+  const int lane = threadIdx.x & 63;
+  double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b0 = 0; b0 < nb; b0 += 512) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int b = b0 + 64 * r + lane;
+      if (b < nb) s[r] += p[(size_t)b * 4 + slot];
+    }
+  }
+  double t = s[0];
+#pragma unroll
+  for (int r = 1; r < 8; ++r) t += s[r];
+  return wsum(t);
+}
+
 __global__ void k_finalize(double *__restrict__ acc, ReduceJob jobs) {
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (wid >= jobs.count) return;
-  const double *p = jobs.part[wid];
-  const int nb = jobs.nblocks[wid];
-  const int slot = jobs.slot[wid];
-  double s = 0.0;
-  for (int b = lane; b < nb; b += 64) s += p[(size_t)b * 4 + slot];
-  s = wsum(s);
+  const double s = wave_slot_sum(jobs.part[wid], jobs.nblocks[wid], jobs.slot[wid]);
   if (lane == 0) acc[jobs.dst[wid]] += s;
+}
+
+// Ensemble form: one wave per job of a device job table.
+__global__ void k_finalize_batch(const BatchJob *__restrict__ jobs, int njobs) {
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= njobs) return;
+  const BatchJob job = jobs[j];
+  const double s = wave_slot_sum(job.part, job.nblocks, job.slot);
+  if ((threadIdx.x & 63) == 0) *job.dst += s;
+}
+
+hipError_t finalize_batch(const BatchJob *jobs, int njobs, hipStream_t st) {
+  (void)hipGetLastError();
+  if (njobs == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize_batch, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs);
+  return hipGetLastError();
 }
 
 hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st) {

@@ -13,106 +13,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fall3d_ads.h"
-#include "ads_aux.h"
-#include "ads_internal.h"
+#include "ads_context.h"
 
 using namespace f3d;
 
-namespace {
-
-enum { KID_SOURCE = 0, KID_SWEEP_X = 1, KID_SWEEP_Y = 2, KID_SWEEP_Z = 3, KID_REDUCE = 4, KID_COUNT = 5 };
-
-// device accumulator layout
-enum { ACC_FLUX = 0, ACC_LEDGER = 6, ACC_INJECTED = 12, ACC_SCRATCH = 13, ACC_LEN = 16 };
-
-struct Tiling {
-  int cpt = 0;      // cells per thread
-  int S = 0;        // segments per line
-  int TX = 0;       // lines per workgroup (y, z) / lines per workgroup (x)
-  int block = 0;    // threads per workgroup
-  int gx = 0;       // workgroups along grid.x
-  int groups = 1;   // bin groups (grid.y)
-  int bpg = 1;      // bins per group
-  int ntrans = 0;   // x tiles (y, z sweeps)
-};
-
-struct PendingEvent {
-  int kid;
-  hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct fall3d_ads_ctx {
-  int nx = 0, ny = 0, nz = 0, nb = 0;
-  int limiter = 1, tm = 2, kind = 8, comp = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  std::string err = "no error";
-
-  // state
-  float *chi = nullptr, *clo = nullptr;
-  double *acum = nullptr;
-  size_t ncell_c = 0;  // (nx+4)(ny+4)(nz+4)nb
-  size_t nacum = 0;    // (nx+4)(ny+4)nb
-
-  // meteo (device layouts, FP32)
-  float *u1 = nullptr, *u2 = nullptr, *v1 = nullptr, *v2 = nullptr, *w1 = nullptr, *w2 = nullptr;
-  float *k1 = nullptr, *k2 = nullptr, *k3 = nullptr, *vs = nullptr;
-  bool meteo_ok = false;
-  bool has_w2 = false;
-
-  // grid
-  double *dxp64[3] = {nullptr, nullptr, nullptr};
-  double *dxb64[3] = {nullptr, nullptr, nullptr};
-  float *hdxp[3] = {nullptr, nullptr, nullptr};
-  float *idxb[3] = {nullptr, nullptr, nullptr};
-  float *dtib[3] = {nullptr, nullptr, nullptr};   // dt/dxb for dtib_dt
-  float *hdtib[3] = {nullptr, nullptr, nullptr};  // 0.5 dt/dxb
-  double dtib_dt = 0.0;
-  bool dtib_ok = false;
-  std::vector<double> hdxp64[3], hdxb64[3];
-  bool grid_ok = false;
-
-  // sparse source
-  int nsrc = 0;
-  size_t *src_idx = nullptr;
-  double *src_rate = nullptr, *src_meas = nullptr;
-  double *src_part = nullptr;
-
-  // accumulators and reduction scratch
-  double *acc = nullptr;
-  double *part[4] = {nullptr, nullptr, nullptr, nullptr};  // source, x, y, z
-  size_t part_cap[4] = {0, 0, 0, 0};                        // in blocks
-  double *red_part = nullptr;
-  int red_cap = 0;
-
-  // staging
-  void *dstage = nullptr;
-  size_t dstage_bytes = 0;
-  void *hu = nullptr, *hv = nullptr, *hw = nullptr;  // CB%u, CB%v, CB%w host arrays
-
-  // tiling
-  Tiling tl[3];
-  int user_cpt[3] = {0, 0, 0}, user_tx[3] = {0, 0, 0}, user_bpg[3] = {0, 0, 0};
-
-  // graph replay of step pairs (even + odd iteration)
-  bool use_graph = true;
-  hipGraphExec_t gexec = nullptr;
-  double g_dt = -1.0, g_stime = -1.0;
-  int g_version = -1;
-  int version = 0;  // bumped whenever launch parameters of a step change
-
-  // profiling
-  bool prof = false;
-  std::vector<PendingEvent> pending;
-  std::vector<hipEvent_t> evpool;
-  double kt_ms[KID_COUNT] = {0, 0, 0, 0, 0};
-  long long kt_n[KID_COUNT] = {0, 0, 0, 0, 0};
-};
-
-namespace {
+namespace f3d {
+namespace host {
 
 int fail(fall3d_ads_ctx *c, int code, const char *fmt, ...) {
   char buf[512];
@@ -123,25 +29,6 @@ int fail(fall3d_ads_ctx *c, int code, const char *fmt, ...) {
   if (c) c->err = buf;
   return code;
 }
-
-#define HIPCHK(ctx, expr)                                                                              \
-  do {                                                                                                 \
-    hipError_t e_ = (expr);                                                                            \
-    if (e_ != hipSuccess) return fail((ctx), FALL3D_ADS_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-hipError_t dalloc(T **p, size_t n) {
-  *p = nullptr;
-  if (n == 0) n = 1;
-  return hipMalloc((void **)p, n * sizeof(T));
-}
-
-void dfree(void *p) {
-  if (p) (void)hipFree(p);
-}
-
-int axis_n(const fall3d_ads_ctx *c, int dir) { return dir == 0 ? c->nx : (dir == 1 ? c->ny : c->nz); }
 
 int ensure_stage(fall3d_ads_ctx *c, size_t bytes) {
   if (c->dstage_bytes >= bytes) return FALL3D_ADS_OK;
@@ -240,12 +127,12 @@ Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
     int bpg = c->user_bpg[dir] > 0 ? std::min(c->user_bpg[dir], 8) : 8;
     int groups = (c->nb + bpg - 1) / bpg;
     bpg = (c->nb + groups - 1) / groups;
-    t.cpt = 1;
+    t.cpt = (c->tm == 1) ? 2 : 1;  // bins per thread (two only for Euler; RK4 would spill)
     t.S = 1;
     t.TX = 64;
     t.bpg = bpg;
     t.groups = groups;
-    t.block = 64 * bpg;
+    t.block = 64 * ((bpg + t.cpt - 1) / t.cpt);
     t.ntrans = (c->nx + 63) / 64;
     t.gx = groups * t.ntrans * (dir == 1 ? c->nz : c->ny);
   }
@@ -281,7 +168,7 @@ struct SweepOut {
 int ensure_dt(fall3d_ads_ctx *c, double dt) {
   if (c->dtib_ok && c->dtib_dt == dt) return FALL3D_ADS_OK;
   for (int d = 0; d < 3; ++d) {
-    hipError_t e = dt_over_dxb(c->dxb64[d], axis_n(c, d) + 3, dt, c->dtib[d], c->hdtib[d], c->stream);
+    hipError_t e = dt_over_dxb(c->dxb64[d], axis_n(c, d) + 3, dt, c->tm, c->dttab[d], c->stream);
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "dt/dxb: %s", hipGetErrorString(e));
   }
   c->dtib_dt = dt;
@@ -289,14 +176,9 @@ int ensure_dt(fall3d_ads_ctx *c, double dt) {
   return FALL3D_ADS_OK;
 }
 
-int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *vel1, const float *vel2,
-              const float *kd, const float *vsd, int write_ghosts, SweepOut *out) {
-  {
-    const int rc = ensure_dt(c, dt);
-    if (rc) return rc;
-  }
+void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int write_ghosts, const float *vel1,
+                     const float *vel2, const float *kd, SweepArgs &a) {
   const Tiling &t = c->tl[dir];
-  SweepArgs a;
   memset(&a, 0, sizeof(a));
   a.chi = c->chi;
   a.clo = c->comp ? c->clo : nullptr;
@@ -319,11 +201,15 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   a.vel1 = vel1;
   a.vel2 = vel2;
   a.kd = kd;
-  a.vs = vsd;
+  a.vs = c->vs;
   a.hdxp = c->hdxp[dir];
   a.idxb = c->idxb[dir];
-  a.dtib = c->dtib[dir];
-  a.hdtib = c->hdtib[dir];
+  {
+    const int n3 = axis_n(c, dir) + 3;
+    a.dtib = c->dttab[dir];
+    a.hdtib = c->dttab[dir] + n3;
+    a.ftib = c->dttab[dir] + 2 * n3;
+  }
   a.dxb64 = c->dxb64[dir];
   if (dir == 0) {
     a.areaA = c->dxb64[1];
@@ -337,6 +223,28 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   }
   a.acum = c->acum;
   a.partials = c->part[dir + 1];
+  if (c->fout_on && dir < 2) {
+    a.fout = c->fout[dir];
+    a.fscale = (float)(c->tm == 1 ? dt : dt / 6.0);
+  }
+}
+
+void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int write_ghosts, SweepArgs &a) {
+  const float *v1 = dir == 0 ? c->u1 : (dir == 1 ? c->v1 : c->w1);
+  const float *v2 = dir == 0 ? c->u2 : (dir == 1 ? c->v2 : c->w2);
+  const float *kd = dir == 0 ? c->k1 : (dir == 1 ? c->k2 : c->k3);
+  make_sweep_args(c, dir, dt, stime, write_ghosts, v1, v2, kd, a);
+}
+
+int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *vel1, const float *vel2,
+              const float *kd, int write_ghosts, SweepOut *out) {
+  {
+    const int rc = ensure_dt(c, dt);
+    if (rc) return rc;
+  }
+  const Tiling &t = c->tl[dir];
+  SweepArgs a;
+  make_sweep_args(c, dir, dt, stime, write_ghosts, vel1, vel2, kd, a);
   dim3 grid = (dir == 0) ? dim3(t.gx, t.groups) : dim3(t.gx);
   dim3 block(t.block);
   {
@@ -530,7 +438,7 @@ int dropin_sweep(fall3d_ads_ctx *c, int dir, double *fbeg, double *fend, double 
   double pair[2] = {*fbeg, *fend};
   if ((rc = set_acc(c, ACC_FLUX + 2 * dir, pair, 2))) return rc;
   SweepOut so;
-  if ((rc = run_sweep(c, dir, dt, stime, vel1, vel2, kdd, c->vs, 1, &so))) return rc;
+  if ((rc = run_sweep(c, dir, dt, stime, vel1, vel2, kdd, 1, &so))) return rc;
   ReduceJob j;
   memset(&j, 0, sizeof(j));
   add_sweep_jobs(j, c, dir, so.nblocks);
@@ -545,7 +453,10 @@ int dropin_sweep(fall3d_ads_ctx *c, int dir, double *fbeg, double *fend, double 
   return FALL3D_ADS_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace f3d
+
+using namespace f3d::host;
 
 // =============================================================================
 // C ABI
@@ -610,8 +521,7 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     HIPCHK(c, dalloc(&c->dxb64[d], n + 3));
     HIPCHK(c, dalloc(&c->hdxp[d], n + 3));
     HIPCHK(c, dalloc(&c->idxb[d], n + 3));
-    HIPCHK(c, dalloc(&c->dtib[d], n + 3));
-    HIPCHK(c, dalloc(&c->hdtib[d], n + 3));
+    HIPCHK(c, dalloc(&c->dttab[d], 3 * (n + 3)));
   }
   const size_t ksz = (size_t)real_kind;
   c->hu = calloc(n_u(c), ksz);
@@ -633,6 +543,7 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
   }
   for (auto e : c->evpool) (void)hipEventDestroy(e);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  learned_release(c);
   void *ptrs[] = {c->chi, c->clo, c->acum, c->u1, c->u2, c->v1, c->v2, c->w1, c->w2, c->k1, c->k2, c->k3, c->vs,
                   c->src_idx, c->src_rate, c->src_meas, c->src_part, c->acc, c->part[1], c->part[2],
                   c->part[3], c->red_part, c->dstage};  // part[0] aliases src_part
@@ -642,8 +553,7 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
     dfree(c->dxb64[d]);
     dfree(c->hdxp[d]);
     dfree(c->idxb[d]);
-    dfree(c->dtib[d]);
-    dfree(c->hdtib[d]);
+    dfree(c->dttab[d]);
   }
   free(c->hu);
   free(c->hv);
@@ -876,9 +786,14 @@ int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
 }
 
 int fall3d_ads_set_state(fall3d_ads_ctx *c, const void *cst, const void *acum) {
-  if (!c || !cst) return FALL3D_ADS_EINVAL;
+  if (!c) return FALL3D_ADS_EINVAL;
   int rc;
-  if ((rc = up_state(c, cst))) return rc;
+  if (cst) {
+    if ((rc = up_state(c, cst))) return rc;
+  } else {  // zero field, no host transfer
+    HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
+    if (c->comp) HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_c * sizeof(float), c->stream));
+  }
   if (acum) {
     if ((rc = up_acum(c, acum))) return rc;
   } else {
@@ -927,6 +842,7 @@ int fall3d_ads_step(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
   int rc;
   if ((rc = need_grid(c))) return rc;
   if (!c->meteo_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_meteo has not been called");
+  if (c->learned && (rc = learned_pre_step(c, dt, stime))) return rc;
   ReduceJob j;
   memset(&j, 0, sizeof(j));
   if (c->nsrc > 0) {
@@ -950,9 +866,10 @@ int fall3d_ads_step(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
     const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
     const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
     const float *kd = d == 0 ? c->k1 : (d == 1 ? c->k2 : c->k3);
-    if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, c->vs, 1, &so))) return rc;
+    if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so))) return rc;
     add_sweep_jobs(j, c, d, so.nblocks);
   }
+  if (c->learned && (rc = learned_post_step(c))) return rc;
   return run_finalize(c, j);
 }
 

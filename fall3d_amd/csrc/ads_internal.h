@@ -41,11 +41,14 @@ struct SweepArgs {
   const float *idxb;         // 1/dxb(p) for cells p = 0..n+2
   const float *dtib;         // dt/dxb(p), p = 0..n+2 (for the launch's dt)
   const float *hdtib;        // 0.5*dt/dxb(p)
+  const float *ftib;         // final-update factor: dt/(6 dxb(p)) for RK4, dt/dxb(p) for Euler
   const double *dxb64;       // dxb(p) in FP64, p = 0..n+2 (mass measure of the ledger)
   const double *areaA;       // FP64 transverse cell sizes for the face area (index 0..)
   const double *areaB;
   double *acum;              // ground accumulation (z sweep only), reference layout, FP64
   double *partials;          // [gridDim.x*gridDim.y][4]: ref flux beg/end, ledger beg/end
+  float *fout;               // optional face mass flux output (x: u layout, y: v layout, per bin), or nullptr
+  float fscale;              // fout += fscale * (G1 + 2 G2 + 2 G3 + G4) (RK4: dt/6; Euler: dt * G)
 };
 
 // One launch over the members of an ensemble: member m owns the workgroups
@@ -56,10 +59,12 @@ struct SweepBatch {
   int nmem;
 };
 
-// dt/dxb and 0.5 dt/dxb for one axis (ads_aux.hip)
-hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, float *dtib, float *hdtib, hipStream_t st);
+// Table of one axis for time step dt (ads_aux.hip), 3 (n+3) floats:
+// [dt/dxb | 0.5 dt/dxb | w dt/dxb], w = 1/6 (RK4) or 1 (Euler, tm == 1).
+hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, hipStream_t st);
 
-// Sweep launchers (ads_sweep_*.hip). cpt = cells per thread.
+// Sweep launchers (ads_sweep_*.hip). cpt = cells per lane (x sweep) or
+// bins per thread (y and z sweeps, 1 or 2).
 hipError_t launch_sweep(int dir, int cpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid,
                         dim3 block, hipStream_t st);
 hipError_t launch_sweep_batch(int dir, int cpt, int limiter, bool comp, int nst, const SweepBatch &b,

@@ -160,40 +160,55 @@ __device__ __forceinline__ void bstore(rsrc_t r, uint32_t voff, uint32_t soff, f
 }
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-constexpr int kPF = 4;  // prefetch distance (cells)
+#ifndef F3D_STREAM_PREFETCH
+#define F3D_STREAM_PREFETCH 4
+#endif
+// prefetch distance (cells); the 8-deep face rings need kPF <= 6
+constexpr int kPF = F3D_STREAM_PREFETCH;
+static_assert(kPF >= 2 && kPF <= 6, "prefetch distance outside the ring capacity");
 
-template <int DIR, int LIM, bool COMP, int NST>
+template <int DIR, int LIM, bool COMP, int NST, int BPT, bool FOUT = false>
 __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   // the wave index is uniform across the wave: say so, so that the per-wave
   // buffer descriptors live in SGPRs
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nx = a.nx, ny = a.ny, nz = a.nz, n = a.n;
-  // block -> (bin group, x tile, transverse line index); wave -> bin
+  // block -> (bin group, x tile, transverse line index); wave -> BPT bins
   const int groups = a.ntrans2;
   const int g = bid % groups;
   const int rest = bid / groups;
   const int tile = rest % a.ntrans;
   const int t2 = rest / a.ntrans + 1;  // k (y sweep) or j (z sweep)
   const int i = tile * 64 + lane + 1;
-  const int b = g * a.nbin + wave;
-  const bool wave_ok = (wave < a.nbin) && (b < a.nbins);
-  const bool act = wave_ok && (i <= nx);
   const int ia = (i <= nx) ? i : nx;  // idle lanes address a valid column
-  const int bb = wave_ok ? b : 0;
+  int bins[BPT];
+  bool bok[BPT];
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    const int loc = wave * BPT + j;  // bin within the group
+    bok[j] = (loc < a.nbin) && (g * a.nbin + loc < a.nbins);
+    bins[j] = bok[j] ? g * a.nbin + loc : 0;
+  }
 
   const size_t cbin = (size_t)(nx + 4) * (ny + 4) * (nz + 4);
-  const rsrc_t rc = make_rsrc(a.chi + cbin * bb, cbin * 4);
-  const rsrc_t rl = make_rsrc(COMP ? a.clo + cbin * bb : a.chi + cbin * bb, cbin * 4);
   const size_t nvel = (DIR == 1) ? (size_t)nx * (ny + 3) * nz : (size_t)nx * ny * (nz + 3);
   const size_t nkd = (DIR == 1) ? (size_t)nx * (ny + 4) * nz : (size_t)nx * ny * (nz + 4);
+  const size_t nvs = (size_t)nx * ny * (nz + 3);
+  rsrc_t rc[BPT], rl[BPT], rvs[BPT], rf[BPT];
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    if constexpr (FOUT) rf[j] = make_rsrc(a.fout + nvel * bins[j], nvel * 4);
+    rc[j] = make_rsrc(a.chi + cbin * bins[j], cbin * 4);
+    rl[j] = make_rsrc(COMP ? a.clo + cbin * bins[j] : a.chi + cbin * bins[j], cbin * 4);
+    rvs[j] = make_rsrc(DIR == 2 ? a.vs + nvs * bins[j] : a.vel1, (DIR == 2 ? nvs : nvel) * 4);
+  }
   const rsrc_t rv1 = make_rsrc(a.vel1, nvel * 4);
   const rsrc_t rv2 = make_rsrc(a.vel2, nvel * 4);
   const rsrc_t rk = make_rsrc(a.kd, nkd * 4);
-  const size_t nvs = (size_t)nx * ny * (nz + 3);
-  const rsrc_t rvs = make_rsrc(DIR == 2 ? a.vs + nvs * bb : a.vel1, (DIR == 2 ? nvs : nvel) * 4);
   cfloat *hdxp = (cfloat *)a.hdxp;
   cfloat *dtib = (cfloat *)a.dtib;
   cfloat *hdtib = (cfloat *)a.hdtib;
+  cfloat *ftib = (cfloat *)a.ftib;
 
   // lane byte offsets of position -1 (c), face 0 (winds), cell -1 (K); byte strides
   uint32_t vc, cs4, vv, vst4, vk, kst4;
@@ -212,74 +227,98 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
     vk = vv;
     kst4 = vst4;
   }
-  const uint32_t vstore = act ? vc : kOOB;
-  const double area = act ? a.areaB[t2] * a.areaA[ia] : 0.0;  // dX3_b(k) dX1_b(i) / dX2_b(j) dX1_b(i)
+  const bool lane_ok = (i <= nx);
+  uint32_t vstore[BPT];
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) vstore[j] = (lane_ok && bok[j]) ? vc : kOOB;
+  const double area = lane_ok ? a.areaB[t2] * a.areaA[ia] : 0.0;  // dX3_b(k) dX1_b(i) / dX2_b(j) dX1_b(i)
   const float s = a.stime, s1 = 1.0f - a.stime;
 
   auto cpos = [&](int P) -> uint32_t { return (uint32_t)clampi(P + 1, 0, n + 3) * cs4; };   // cell P of c
   auto fpos = [&](int f) -> uint32_t { return (uint32_t)clampi(f, 0, n + 2) * vst4; };      // face f
   auto kpos = [&](int P) -> uint32_t { return (uint32_t)clampi(P + 1, 0, n + 3) * kst4; };  // cell P of K
-  auto face_v = [&](int f) -> float {  // wind (w_eff for z) at face f of this line
-    float v = s1 * bload(rv1, vv, fpos(f)) + s * bload(rv2, vv, fpos(f));
-    if constexpr (DIR == 2) v -= bload(rvs, vv, fpos(f));
-    return v;
-  };
 
   // free-flow ghosts from cells 1, n and faces 1, n+1 (Sources/mod_ADS.F90:1239-1324)
-  const float c1 = bload(rc, vc, cpos(1));
-  const float cnn = bload(rc, vc, cpos(n));
-  const float vfirst = face_v(1), vlast = face_v(n + 1);
-  const float gL = (vfirst > 0.0f) ? (2.0f * 0.0f - c1) : c1;
-  const float gR = (vlast < 0.0f) ? (2.0f * 0.0f - cnn) : cnn;
+  float gL[BPT], gR[BPT];
   {
-    const uint32_t vg = a.write_ghosts ? vstore : kOOB;
-    bstore(rc, vg, cpos(-1), gL);
-    bstore(rc, vg, cpos(0), gL);
-    bstore(rc, vg, cpos(n + 1), gR);
-    bstore(rc, vg, cpos(n + 2), gR);
-    if constexpr (COMP) {
-      bstore(rl, vg, cpos(-1), 0.0f);
-      bstore(rl, vg, cpos(0), 0.0f);
-      bstore(rl, vg, cpos(n + 1), 0.0f);
-      bstore(rl, vg, cpos(n + 2), 0.0f);
+    const float wb = s1 * bload(rv1, vv, fpos(1)) + s * bload(rv2, vv, fpos(1));
+    const float we = s1 * bload(rv1, vv, fpos(n + 1)) + s * bload(rv2, vv, fpos(n + 1));
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      float vfirst = wb, vlast = we;
+      if constexpr (DIR == 2) {
+        vfirst -= bload(rvs[j], vv, fpos(1));
+        vlast -= bload(rvs[j], vv, fpos(n + 1));
+      }
+      const float c1 = bload(rc[j], vc, cpos(1));
+      const float cnn = bload(rc[j], vc, cpos(n));
+      gL[j] = (vfirst > 0.0f) ? (2.0f * 0.0f - c1) : c1;
+      gR[j] = (vlast < 0.0f) ? (2.0f * 0.0f - cnn) : cnn;
+      const uint32_t vg = a.write_ghosts ? vstore[j] : kOOB;
+      bstore(rc[j], vg, cpos(-1), gL[j]);
+      bstore(rc[j], vg, cpos(0), gL[j]);
+      bstore(rc[j], vg, cpos(n + 1), gR[j]);
+      bstore(rc[j], vg, cpos(n + 2), gR[j]);
+      if constexpr (COMP) {
+        bstore(rl[j], vg, cpos(-1), 0.0f);
+        bstore(rl[j], vg, cpos(0), 0.0f);
+        bstore(rl[j], vg, cpos(n + 1), 0.0f);
+        bstore(rl[j], vg, cpos(n + 2), 0.0f);
+      }
     }
   }
 
-  // rings: cells (16), low words (8, fetched kPF ahead of the final update),
-  // raw face data (8) and face coefficients (8)
-  float cn[16], lo[8];
-  float rw1[8], rw2[8], rws[8], rkk[8];
-  float up[8], um[8], kf[8], gh[8];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) cn[q] = 0.0f;
+  // rings: cells (16) and low words (8, fetched kPF ahead of the final
+  // update) per bin; raw face data (8) and face coefficients (8). The wind
+  // lerp and K are shared by the bins of a thread; in the z sweep the
+  // settling velocity and hence the upwind coefficients are per bin.
+  constexpr int BV = (DIR == 2) ? BPT : 1;
+  float cn[BPT][16], lo[BPT][8], gh[BPT][8];
+  float rw1[8], rw2[8], rkk[8], kf[8];
+  float rws[BV][8], up[BV][8], um[BV][8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    rw1[q] = rw2[q] = rws[q] = rkk[q] = 0.0f;
-    up[q] = um[q] = kf[q] = gh[q] = lo[q] = 0.0f;
+    rw1[q] = rw2[q] = rkk[q] = kf[q] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < BV; ++j) rws[j][q] = up[j][q] = um[j][q] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) gh[j][q] = lo[j][q] = 0.0f;
   }
+#pragma unroll
+  for (int j = 0; j < BPT; ++j)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) cn[j][q] = 0.0f;
   // per-stage pipeline state: previous two inputs, last difference, last
   // slope, last face flux. Every stage starts as if it had consumed an
   // unbounded run of west ghosts (value gL: zero difference and slope).
-  float sa1[NST], sa2[NST], sd[NST], sm[NST], sG[NST];
+  float sa1[BPT][NST], sa2[BPT][NST], sd[BPT][NST], sm[BPT][NST], sG[BPT][NST];
 #pragma unroll
-  for (int st = 0; st < NST; ++st) {
-    sa1[st] = sa2[st] = gL;
-    sd[st] = sm[st] = sG[st] = 0.0f;
-  }
+  for (int j = 0; j < BPT; ++j)
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      sa1[j][st] = sa2[j][st] = gL[j];
+      sd[j][st] = sm[j][st] = sG[j][st] = 0.0f;
+    }
   // ring slots: cell p -> p & 15, face f -> f & 7. Preload what iterations
   // t = -kPF..-1 would have fetched (cells -1..kPF-2, faces -2..kPF-3), so
   // that the pipeline starts at t = 0.
 #pragma unroll
-  for (int p = -1; p <= kPF - 2; ++p) cn[p & 15] = bload(rc, vc, cpos(p));
+  for (int p = -1; p <= kPF - 2; ++p)
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) cn[j][p & 15] = bload(rc[j], vc, cpos(p));
 #pragma unroll
   for (int f = -2; f <= kPF - 3; ++f) {
     rw1[f & 7] = bload(rv1, vv, fpos(f));
     rw2[f & 7] = bload(rv2, vv, fpos(f));
-    if constexpr (DIR == 2) rws[f & 7] = bload(rvs, vv, fpos(f));
     rkk[f & 7] = bload(rk, vk, kpos(f));
+    if constexpr (DIR == 2)
+#pragma unroll
+      for (int j = 0; j < BV; ++j) rws[j][f & 7] = bload(rvs[j], vv, fpos(f));
   }
 
-  float RFb = 0.0f, RFe = 0.0f, tl1 = 0.0f, trn = 0.0f;
+  float RFb[BPT], RFe[BPT], tl1[BPT], trn[BPT];
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) RFb[j] = RFe[j] = tl1[j] = trn[j] = 0.0f;
   const int tend = n + 2 * NST + 1;  // iteration of the last final output (cell n)
 
   // One pipeline step at iteration t (u = t mod 16, a compile-time constant
@@ -289,94 +328,119 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   auto step = [&](auto gen_tag, const int t, const int u) {
     constexpr bool GEN = decltype(gen_tag)::value;
     // ---- prefetch cell t+kPF-1 and face t+kPF-2
-    cn[(u + kPF - 1) & 15] = bload(rc, vc, cpos(t + kPF - 1));
-    if constexpr (COMP) lo[(u + kPF - 2 * NST - 1) & 7] = bload(rl, vc, cpos(t + kPF - 2 * NST - 1));
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      cn[j][(u + kPF - 1) & 15] = bload(rc[j], vc, cpos(t + kPF - 1));
+      if constexpr (COMP) lo[j][(u + kPF - 2 * NST - 1) & 7] = bload(rl[j], vc, cpos(t + kPF - 2 * NST - 1));
+    }
     {
       const int f = t + kPF - 2;
+      const int sl = (u + kPF - 2) & 7;
       const uint32_t fo = fpos(f);
-      rw1[(u + kPF - 2) & 7] = bload(rv1, vv, fo);
-      rw2[(u + kPF - 2) & 7] = bload(rv2, vv, fo);
-      if constexpr (DIR == 2) rws[(u + kPF - 2) & 7] = bload(rvs, vv, fo);
-      rkk[(u + kPF - 2) & 7] = bload(rk, vk, kpos(f));
+      rw1[sl] = bload(rv1, vv, fo);
+      rw2[sl] = bload(rv2, vv, fo);
+      rkk[sl] = bload(rk, vk, kpos(f));
+      if constexpr (DIR == 2)
+#pragma unroll
+        for (int j = 0; j < BV; ++j) rws[j][sl] = bload(rvs[j], vv, fo);
     }
     // ---- coefficients of face t-2 (first used by stage 1 in this iteration)
     {
       const int sl = (u - 2) & 7;
-      float v = s1 * rw1[sl] + s * rw2[sl];
-      if constexpr (DIR == 2) v -= rws[sl];
-      up[sl] = fmaxf(v, 0.0f);
-      um[sl] = fminf(v, 0.0f);
+      const float v = s1 * rw1[sl] + s * rw2[sl];
+#pragma unroll
+      for (int j = 0; j < BV; ++j) {
+        float ve = v;
+        if constexpr (DIR == 2) ve -= rws[j][sl];
+        up[j][sl] = fmaxf(ve, 0.0f);
+        um[j][sl] = fminf(ve, 0.0f);
+      }
       kf[sl] = (rkk[(u - 3) & 7] + rkk[sl]) * hdxp[clampi(t - 2, 0, n + 2)];
     }
     // ---- the pipeline: stage st consumes position t+1-2(st+1)
-    float v;
-    if constexpr (GEN) {
-      const int P = t - 1;
-      v = (P <= 0) ? gL : ((P > n) ? gR : cn[(u - 1) & 15]);
-    } else {
-      v = cn[(u - 1) & 15];
-    }
 #pragma unroll
-    for (int st = 0; st < NST; ++st) {
-      const int f = t - 2 * (st + 1);  // face completed by this stage
-      const int fs = (u - 2 * (st + 1)) & 7;
-      const float d = v - sa1[st];
-      const float m = limiter_slope<LIM>(sd[st], d);
-      const float cL = fmaf(0.5f, sm[st], sa2[st]);
-      const float cR = fmaf(-0.5f, m, sa1[st]);
-      const float F = fmaf(up[fs], cL, um[fs] * cR);  // KT flux with a = |u| (upwind)
-      const float Pd = kf[fs] * sd[st];
-      const float G = F - Pd;
-      float wgt;
-      if constexpr (NST == 1)
-        wgt = 1.0f;
-      else
-        wgt = (st == 0 || st == 3) ? (1.0f / 6.0f) : (1.0f / 3.0f);
-      gh[fs] = (st == 0) ? wgt * G : fmaf(wgt, G, gh[fs]);
+    for (int j = 0; j < BPT; ++j) {
+      const int jv = (DIR == 2) ? j : 0;
+      float v;
       if constexpr (GEN) {
-        if (st == NST - 1) {  // reference end-face fluxes F + P of the last stage
-          RFb = (f == 1) ? F + Pd : RFb;
-          RFe = (f == n + 1) ? F + Pd : RFe;
+        const int P = t - 1;
+        v = (P <= 0) ? gL[j] : ((P > n) ? gR[j] : cn[j][(u - 1) & 15]);
+      } else {
+        v = cn[j][(u - 1) & 15];
+      }
+#pragma unroll
+      for (int st = 0; st < NST; ++st) {
+        const int f = t - 2 * (st + 1);  // face completed by this stage
+        const int fs = (u - 2 * (st + 1)) & 7;
+        const float d = v - sa1[j][st];
+        const float m = limiter_slope<LIM>(sd[j][st], d);
+        const float cL = fmaf(0.5f, sm[j][st], sa2[j][st]);
+        const float cR = fmaf(-0.5f, m, sa1[j][st]);
+        // G = F - P with the KT flux F = max(u,0) cL + min(u,0) cR (a = |u|: upwind)
+        const float Pd = kf[fs] * sd[j][st];
+        const float G = fmaf(up[jv][fs], cL, fmaf(um[jv][fs], cR, -Pd));
+        // RK4 weights 1 2 2 1 (the 1/6 is in ftib)
+        if (st == 0)
+          gh[j][fs] = G;
+        else if (st == NST - 1)
+          gh[j][fs] = gh[j][fs] + G;
+        else
+          gh[j][fs] = fmaf(2.0f, G, gh[j][fs]);
+        if constexpr (GEN) {
+          if (st == NST - 1) {  // reference end-face fluxes F + P of the last stage
+            RFb[j] = (f == 1) ? fmaf(2.0f, Pd, G) : RFb[j];
+            RFe[j] = (f == n + 1) ? fmaf(2.0f, Pd, G) : RFe[j];
+          }
+        }
+        const int q = f - 1;  // output cell of this stage
+        const float Gp = sG[j][st];
+        sa2[j][st] = sa1[j][st];
+        sa1[j][st] = v;
+        sd[j][st] = d;
+        sm[j][st] = m;
+        sG[j][st] = G;
+        if (st < NST - 1) {
+          const float hib = (st == 2) ? dtib[clampi(q, 0, n + 2)] : hdtib[clampi(q, 0, n + 2)];
+          const float yn = fmaf(hib, Gp - G, cn[j][(u - 2 * (st + 1) - 1) & 15]);
+          if constexpr (GEN)
+            v = (q <= 0) ? gL[j] : ((q > n) ? gR[j] : yn);  // next stage input (position q)
+          else
+            v = yn;
         }
       }
-      const int q = f - 1;  // output cell of this stage
-      const float Gp = sG[st];
-      sa2[st] = sa1[st];
-      sa1[st] = v;
-      sd[st] = d;
-      sm[st] = m;
-      sG[st] = G;
-      if (st < NST - 1) {
-        const float hib = (st == 2) ? dtib[clampi(q, 0, n + 2)] : hdtib[clampi(q, 0, n + 2)];
-        const float yn = fmaf(hib, Gp - G, cn[(u - 2 * (st + 1) - 1) & 15]);
-        if constexpr (GEN)
-          v = (q <= 0) ? gL : ((q > n) ? gR : yn);  // next stage input (position q)
-        else
-          v = yn;
+      // ---- final update of cell q = t - 2 NST - 1
+      {
+        const int q = t - 2 * NST - 1;
+        const int cl = (u - 2 * NST - 1) & 15;
+        const float dtq = ftib[clampi(q, 0, n + 2)];
+        const float gq = gh[j][(u - 2 * NST - 1) & 7], gq1 = gh[j][(u - 2 * NST) & 7];
+        uint32_t vo = vstore[j];
+        if constexpr (GEN) {
+          vo = (q >= 1 && q <= n) ? vstore[j] : kOOB;
+          tl1[j] = (q == 1) ? dtq * gq : tl1[j];
+          trn[j] = (q == n) ? dtq * gq1 : trn[j];
+        }
+        if constexpr (COMP) {
+          float hv = cn[j][cl], lv = lo[j][(u - 2 * NST - 1) & 7];
+          comp_add(hv, lv, dtq * gq);
+          comp_add(hv, lv, -(dtq * gq1));
+          bstore(rc[j], vo, cpos(q), hv);
+          bstore(rl[j], vo, cpos(q), lv);
+        } else {
+          bstore(rc[j], vo, cpos(q), fmaf(dtq, gq - gq1, cn[j][cl]));
+        }
+        if constexpr (FOUT) {  // face mass flux per unit area: faces q (1..n) and n+1
+          const uint32_t fo = fpos(q), fo1 = fpos(q + 1);
+          const uint32_t vf = (vo != kOOB) ? vv : kOOB;  // same predicate, face-array lane offset
+          bstore(rf[j], vf, fo, fmaf(a.fscale, gq, bload(rf[j], vv, fo)));
+          const uint32_t ve = (q == n) ? vf : kOOB;
+          bstore(rf[j], ve, fo1, fmaf(a.fscale, gq1, bload(rf[j], vv, fo1)));
+        }
       }
     }
-    // ---- final update of cell q = t - 2 NST - 1
-    {
-      const int q = t - 2 * NST - 1;
-      const int cl = (u - 2 * NST - 1) & 15;
-      const float dtq = dtib[clampi(q, 0, n + 2)];
-      const float gq = gh[(u - 2 * NST - 1) & 7], gq1 = gh[(u - 2 * NST) & 7];
-      uint32_t vo = vstore;
-      if constexpr (GEN) {
-        vo = (q >= 1 && q <= n) ? vstore : kOOB;
-        tl1 = (q == 1) ? dtq * gq : tl1;
-        trn = (q == n) ? dtq * gq1 : trn;
-      }
-      if constexpr (COMP) {
-        float hv = cn[cl], lv = lo[(u - 2 * NST - 1) & 7];
-        comp_add(hv, lv, dtq * gq);
-        comp_add(hv, lv, -(dtq * gq1));
-        bstore(rc, vo, cpos(q), hv);
-        bstore(rl, vo, cpos(q), lv);
-      } else {
-        bstore(rc, vo, cpos(q), fmaf(dtq, gq - gq1, cn[cl]));
-      }
-    }
+    // keep the scheduler from hoisting the scalar loads and offsets of all
+    // 16 unrolled iterations to the block top (SGPR spills to VGPR lanes)
+    __builtin_amdgcn_sched_barrier(0);
   };
 
   // Blocks of 16 iterations. Interior blocks (every t in [2 NST + 3, n + 1]:
@@ -396,40 +460,55 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   }
 
   double rb = 0.0, re = 0.0, lb = 0.0, le = 0.0;
-  if (act) {
-    const float dt = a.dt;
-    rb = (double)RFb * area * (double)dt;
-    re = (double)RFe * area * (double)dt;
-    lb = -(double)tl1 * a.dxb64[1] * area;
-    le = (double)trn * a.dxb64[n] * area;
-    if constexpr (DIR == 2) {
-      const size_t ai = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(t2 + 1) + (size_t)(ny + 4) * bb);
-      a.acum[ai] = a.acum[ai] + fabs((double)RFb) * (double)dt;
+  if (lane_ok) {
+    const double dt = (double)a.dt;
+#pragma unroll
+    for (int j = 0; j < BPT; ++j) {
+      if (!bok[j]) continue;
+      rb += (double)RFb[j] * area * dt;
+      re += (double)RFe[j] * area * dt;
+      lb -= (double)tl1[j] * a.dxb64[1] * area;
+      le += (double)trn[j] * a.dxb64[n] * area;
+      if constexpr (DIR == 2) {
+        const size_t ai = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(t2 + 1) + (size_t)(ny + 4) * bins[j]);
+        a.acum[ai] = a.acum[ai] + fabs((double)RFb[j]) * dt;
+      }
     }
   }
   block_partials(rb, re, lb, le, a.partials, bid);
 }
 
-template <int DIR, int LIM, bool COMP, int NST>
+// Workgroup order: consecutive workgroups are dispatched round-robin over
+// the 8 XCDs (each with its own L2). Renumber so that each XCD receives a
+// contiguous range of logical blocks: neighbouring x tiles and the bin
+// groups of the same columns then share an L2.
+__device__ __forceinline__ int xcd_block(const int b, const int nblocks) {
+  const int q = nblocks >> 3, r = nblocks & 7;
+  const int xcd = b & 7, idx = b >> 3;
+  return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+template <int DIR, int LIM, bool COMP, int NST, int BPT, bool FOUT = false>
 __global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
-  stream_body<DIR, LIM, COMP, NST>(a, blockIdx.x);
+  stream_body<DIR, LIM, COMP, NST, BPT, FOUT>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // All members of an ensemble in one launch: the workgroup finds its member
 // in the prefix table of workgroup counts, reads that member's arguments
 // (scalar loads) and runs the same body on its member-local block index.
-template <int DIR, int LIM, bool COMP, int NST>
+template <int DIR, int LIM, bool COMP, int NST, int BPT>
 __global__ void __launch_bounds__(512) k_stream_batch(SweepBatch b) {
-  const int m = batch_member(b, blockIdx.x);
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int m = batch_member(b, lb);
   const SweepArgs a = member_args(b, m);
-  stream_body<DIR, LIM, COMP, NST>(a, blockIdx.x - ((cint *)b.bstart)[m]);
+  stream_body<DIR, LIM, COMP, NST, BPT>(a, lb - ((cint *)b.bstart)[m]);
 }
 
 // =============================================================================
 // k_sweep_x: x sweep, one wave per line, C cells per lane
 // =============================================================================
 
-template <int C, int LIM, bool COMP, int NST>
+template <int C, int LIM, bool COMP, int NST, bool FOUT = false>
 __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, const int by) {
   static_assert(C >= 2, "segments need two edge cells");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -456,7 +535,7 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   const float dt = a.dt;
 
   // wind and diffusion of this line, read once for all bins
-  float up[C + 1], um[C + 1], kf[C + 1], ib[C];
+  float up[C + 1], um[C + 1], kf[C + 1], hb[C], db[C], fb[C];
   float vbeg = 0.0f, vend = 0.0f;
   {
     float kprev = act ? a.kd[kbase + (size_t)s0] : 0.0f;  // cell s0-1 (stored at index s0)
@@ -479,7 +558,9 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
     for (int q = 0; q < C; ++q) {
       int p = s0 + q;
       p = p > n + 2 ? n + 2 : p;
-      ib[q] = act ? a.idxb[p] : 0.0f;
+      db[q] = act ? a.dtib[p] : 0.0f;
+      hb[q] = act ? a.hdtib[p] : 0.0f;
+      fb[q] = act ? a.ftib[p] : 0.0f;
     }
     if (act) {
       vbeg = s1 * a.vel1[vbase + 1] + s * a.vel2[vbase + 1];
@@ -558,7 +639,7 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
     for (int q = 0; q < C; ++q) y[q] = cn[q];
 #pragma unroll
     for (int q = 0; q <= C; ++q) gh[q] = 0.0f;
-    float Fb = 0.0f, Pb = 0.0f, Fe = 0.0f, Pe = 0.0f;
+    float RFb = 0.0f, RFe = 0.0f;
 
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
@@ -583,12 +664,6 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       w[C + 2] = yr0;
       w[C + 3] = yr1;
       const bool last_stage = (st == NST - 1);
-      float wgt;
-      if constexpr (NST == 1)
-        wgt = 1.0f;
-      else
-        wgt = (st == 0 || st == 3) ? (1.0f / 6.0f) : (1.0f / 3.0f);
-      const float h = (st == 2) ? dt : 0.5f * dt;
       float d_cur = w[2] - w[1];
       float m_prev = limiter_slope<LIM>(w[1] - w[0], d_cur);
       float G_prev = 0.0f;
@@ -598,22 +673,22 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
         const float m_cur = limiter_slope<LIM>(d_cur, d_next);
         const float cL = fmaf(0.5f, m_prev, w[q + 1]);
         const float cR = fmaf(-0.5f, m_cur, w[q + 2]);
-        const float F = fmaf(up[q], cL, um[q] * cR);
+        // G = F - P with F = max(u,0) cL + min(u,0) cR
         const float P = kf[q] * d_cur;
-        const float G = F - P;
-        if (last_stage) {
-          if (q == 0) {
-            Fb = F;
-            Pb = P;
-          }
-          if (q == qn + 1) {
-            Fe = F;
-            Pe = P;
-          }
+        const float G = fmaf(up[q], cL, fmaf(um[q], cR, -P));
+        if (last_stage) {  // reference boundary fluxes F + P
+          if (q == 0) RFb = fmaf(2.0f, P, G);
+          if (q == qn + 1) RFe = fmaf(2.0f, P, G);
         }
-        gh[q] = (st == 0) ? wgt * G : fmaf(wgt, G, gh[q]);
+        // RK4 weights 1 2 2 1 (the 1/6 is in fb)
+        if (st == 0)
+          gh[q] = G;
+        else if (last_stage)
+          gh[q] = gh[q] + G;
+        else
+          gh[q] = fmaf(2.0f, G, gh[q]);
         if (q > 0 && !last_stage) {
-          const float yn = fmaf(h, (G_prev - G) * ib[q - 1], cn[q - 1]);
+          const float yn = fmaf((st == 2) ? db[q - 1] : hb[q - 1], G_prev - G, cn[q - 1]);
           y[q - 1] = (last && q - 1 > qn) ? gR : yn;
         }
         G_prev = G;
@@ -622,12 +697,20 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       }
     }
 
+    if constexpr (FOUT) {  // face mass flux per unit area: faces s0 .. s0+C-1, and n+1
+      float *fo = a.fout + (size_t)(nx + 3) * ny * nz * b + vbase;
+#pragma unroll
+      for (int q = 0; q <= C; ++q) {
+        const int p = s0 + q;
+        if (act && ((q < C && p <= n + 1) || (q == C && p == n + 1))) fo[p] = fmaf(a.fscale, gh[q], fo[p]);
+      }
+    }
     float tl0 = 0.0f, trn = 0.0f;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int p = s0 + q;
-      const float tl = (dt * gh[q]) * ib[q];
-      const float tr = (dt * gh[q + 1]) * ib[q];
+      const float tl = fb[q] * gh[q];
+      const float tr = fb[q] * gh[q + 1];
       if (q == 0) tl0 = tl;
       if (q == qn) trn = tr;
       if (act && p <= n) {
@@ -638,17 +721,17 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
           a.chi[cb + p] = hv;
           a.clo[cb + p] = lv;
         } else {
-          a.chi[cb + p] = fmaf(dt * (gh[q] - gh[q + 1]), ib[q], cn[q]);
+          a.chi[cb + p] = fmaf(fb[q], gh[q] - gh[q + 1], cn[q]);
         }
       }
     }
     if (act) {
       if (seg == 0) {
-        acc_rb += (double)(Fb + Pb) * area * (double)dt;
+        acc_rb += (double)RFb * area * (double)dt;
         acc_lb -= (double)tl0 * a.dxb64[1] * area;
       }
       if (seg == S - 1) {
-        acc_re += (double)(Fe + Pe) * area * (double)dt;
+        acc_re += (double)RFe * area * (double)dt;
         acc_le += (double)trn * a.dxb64[n] * area;
       }
     }
@@ -656,9 +739,9 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   block_partials(acc_rb, acc_re, acc_lb, acc_le, a.partials, (size_t)by * a.gx + bx);
 }
 
-template <int C, int LIM, bool COMP, int NST>
+template <int C, int LIM, bool COMP, int NST, bool FOUT = false>
 __global__ void __launch_bounds__(256) k_sweep_x(SweepArgs a) {
-  sweep_x_body<C, LIM, COMP, NST>(a, blockIdx.x, blockIdx.y);
+  sweep_x_body<C, LIM, COMP, NST, FOUT>(a, blockIdx.x, blockIdx.y);
 }
 
 template <int C, int LIM, bool COMP, int NST>

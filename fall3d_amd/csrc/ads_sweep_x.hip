@@ -9,6 +9,11 @@ namespace f3d {
 template <int C>
 static hipError_t launch_x_c(int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,
                              hipStream_t st) {
+  if (a.fout) {  // face-flux output: minmod, FP32 state, RK4 only (training-data runs)
+    if (limiter != 1 || comp || nst != 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_sweep_x<C, 1, false, 4, true>), grid, block, 0, st, a);
+    return hipGetLastError();
+  }
 #define F3D_L(L, CP, N)                                                                     \
   if (limiter == L && comp == CP && nst == N) {                                             \
     hipLaunchKernelGGL((k_sweep_x<C, L, CP, N>), grid, block, 0, st, a);                    \

@@ -4,35 +4,40 @@
 
 namespace f3d {
 
-hipError_t launch_sweep_z(int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,
-                            hipStream_t st) {
+// Two bins per thread only for Euler: the RK4 pipeline of two bins does not
+// fit in 256 VGPRs.
+#define F3D_STREAM_CASES(LAUNCH)                                                       \
+  LAUNCH(1, false, 4, 1) LAUNCH(2, false, 4, 1) LAUNCH(3, false, 4, 1)                 \
+  LAUNCH(1, true, 4, 1) LAUNCH(2, true, 4, 1) LAUNCH(3, true, 4, 1)                    \
+  LAUNCH(1, false, 1, 1) LAUNCH(2, false, 1, 1) LAUNCH(3, false, 1, 1)                 \
+  LAUNCH(1, true, 1, 1) LAUNCH(2, true, 1, 1) LAUNCH(3, true, 1, 1)                    \
+  LAUNCH(1, false, 1, 2) LAUNCH(2, false, 1, 2) LAUNCH(3, false, 1, 2)                 \
+  LAUNCH(1, true, 1, 2) LAUNCH(2, true, 1, 2) LAUNCH(3, true, 1, 2)
+
+hipError_t launch_sweep_z(int bpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid,
+                          dim3 block, hipStream_t st) {
   (void)hipGetLastError();
-#define F3D_L(L, CP, N)                                                                     \
-  if (limiter == L && comp == CP && nst == N) {                                             \
-    hipLaunchKernelGGL((k_stream<2, L, CP, N>), grid, block, 0, st, a);                  \
-    return hipGetLastError();                                                               \
+#define F3D_L(L, CP, N, B)                                                             \
+  if (limiter == L && comp == CP && nst == N && bpt == B) {                            \
+    hipLaunchKernelGGL((k_stream<2, L, CP, N, B>), grid, block, 0, st, a);             \
+    return hipGetLastError();                                                          \
   }
-  F3D_L(1, false, 4) F3D_L(2, false, 4) F3D_L(3, false, 4)
-  F3D_L(1, true, 4) F3D_L(2, true, 4) F3D_L(3, true, 4)
-  F3D_L(1, false, 1) F3D_L(2, false, 1) F3D_L(3, false, 1)
-  F3D_L(1, true, 1) F3D_L(2, true, 1) F3D_L(3, true, 1)
+  F3D_STREAM_CASES(F3D_L)
 #undef F3D_L
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_sweep_z_batch(int limiter, bool comp, int nst, const SweepBatch &b, int nblocks, dim3 block,
-                                hipStream_t st) {
+hipError_t launch_sweep_z_batch(int bpt, int limiter, bool comp, int nst, const SweepBatch &b, int nblocks,
+                                dim3 block, hipStream_t st) {
   (void)hipGetLastError();
-#define F3D_L(L, CP, N)                                                                     \
-  if (limiter == L && comp == CP && nst == N) {                                             \
-    hipLaunchKernelGGL((k_stream_batch<2, L, CP, N>), dim3(nblocks), block, 0, st, b);     \
-    return hipGetLastError();                                                               \
+#define F3D_L(L, CP, N, B)                                                             \
+  if (limiter == L && comp == CP && nst == N && bpt == B) {                            \
+    hipLaunchKernelGGL((k_stream_batch<2, L, CP, N, B>), dim3(nblocks), block, 0, st, b); \
+    return hipGetLastError();                                                          \
   }
-  F3D_L(1, false, 4) F3D_L(2, false, 4) F3D_L(3, false, 4)
-  F3D_L(1, true, 4) F3D_L(2, true, 4) F3D_L(3, true, 4)
-  F3D_L(1, false, 1) F3D_L(2, false, 1) F3D_L(3, false, 1)
-  F3D_L(1, true, 1) F3D_L(2, true, 1) F3D_L(3, true, 1)
+  F3D_STREAM_CASES(F3D_L)
 #undef F3D_L
   return hipErrorInvalidValue;
 }
+
 }  // namespace f3d

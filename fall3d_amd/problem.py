@@ -180,24 +180,26 @@ class FourierWind:
         self.ky = kmag * np.sin(theta)
         # equal velocity amplitude per mode: a_m * |k_m| = 1
         self.a = 1.0 / kmag
-        if amp_factors is not None:
-            self.a = self.a * np.asarray(amp_factors)
+        # per-mode perturbation factors (ensembles); applied after the
+        # amplitude calibration, which uses the unperturbed modes
+        self.factors = None if amp_factors is None else np.asarray(amp_factors, dtype=np.float64)
         self.amplitude = amplitude
         self.shear = shear
         self.scale = 1.0
 
-    def _raw(self, x, y):
+    def _raw(self, x, y, perturbed=True):
+        a = self.a * self.factors if (perturbed and self.factors is not None) else self.a
         ph = np.multiply.outer(x, self.kx) + np.multiply.outer(y, self.ky) + self.phi
         cs = np.cos(ph)
-        u = (cs * (self.a * self.ky)).sum(-1)
-        v = -(cs * (self.a * self.kx)).sum(-1)
+        u = (cs * (a * self.ky)).sum(-1)
+        v = -(cs * (a * self.kx)).sum(-1)
         return u, v
 
     def calibrate(self, Lx, Ly, n=96):
         xs = np.linspace(0.0, Lx, n)
         ys = np.linspace(0.0, Ly, n)
         X, Y = np.meshgrid(xs, ys, indexing="ij")
-        u, v = self._raw(X, Y)
+        u, v = self._raw(X, Y, perturbed=False)
         sp = np.sqrt(u * u + v * v).max()
         self.scale = self.amplitude / sp if sp > 0 else 1.0
 
@@ -268,6 +270,63 @@ def operational_problem(nx=360, ny=240, nz=60, nb=10, dx=5000.0, dy=5000.0, dz=3
             "sources": sources}
     return Problem(nx, ny, nz, nb, g, u, u.copy(order="F"), v, v.copy(order="F"), w, w.copy(order="F"),
                    k1, k2, k3, vs, c, s, name=f"oper{nx}x{ny}x{nz}x{nb}s{seed}", meta=meta)
+
+
+def latin_hypercube(n, dims, rng):
+    """n samples in [-1, 1]^dims with exactly one sample per 1/n stratum along
+    every dimension (the reference draws its ensemble perturbations by Latin
+    hypercube sampling, Sources/mod_Ensemble.F90:763-766)."""
+    strata = np.stack([rng.permutation(n) for _ in range(dims)], axis=1)
+    u = (strata + rng.random((n, dims))) / n
+    return 2.0 * u - 1.0
+
+
+@dataclasses.dataclass
+class EnsembleMember:
+    """One ensemble member: grid size and the perturbed operational-problem
+    parameters; problem() builds it (member 0 is the unperturbed run)."""
+    nx: int
+    ny: int
+    nz: int
+    nb: int
+    column_top: float
+    mass_rate: float
+    amp_factors: np.ndarray
+    wind_seed: int
+
+    def problem(self, dtype_c=np.float64) -> "Problem":
+        return operational_problem(self.nx, self.ny, self.nz, self.nb, seed=self.wind_seed,
+                                   column_top=self.column_top, mass_rate=self.mass_rate,
+                                   amp_factors=self.amp_factors, dtype_c=dtype_c)
+
+
+def ensemble_members(nmembers=64, nb=6, seed=3, nxy=(96, 256), nzr=(40, 60), column_top=15000.0,
+                     mass_rate=1.0e6, height_range=0.30, wind_range=0.20, rate_sigma=0.5, nmodes=16):
+    """Ensemble of operational problems of uneven sizes.
+
+    Sizes: nx, ny uniform in nxy, nz uniform in nzr (inclusive). Perturbations
+    follow Sources/mod_Ensemble.F90:888-935 (relative: var (1 + r range),
+    clipped at 0 for the column height): r in [-1, 1] from a Latin hypercube,
+    column height +-height_range, each wind Fourier-mode amplitude
+    +-wind_range; the mass rate is multiplied by exp(rate_sigma z), z standard
+    normal (a log-normal factor). Member 0 is unperturbed, as the reference's
+    first member (Sources/mod_Ensemble.F90:774-781). All members share the
+    base wind (seed) and the settling velocities of operational_problem.
+    """
+    rng = np.random.default_rng(seed)
+    nx = rng.integers(nxy[0], nxy[1] + 1, nmembers)
+    ny = rng.integers(nxy[0], nxy[1] + 1, nmembers)
+    nz = rng.integers(nzr[0], nzr[1] + 1, nmembers)
+    lhs = latin_hypercube(max(nmembers - 1, 1), 1 + nmodes, rng)
+    z = rng.standard_normal(nmembers)
+    out = []
+    for m in range(nmembers):
+        r = np.zeros(1 + nmodes) if m == 0 else lhs[m - 1]
+        top = max(0.0, column_top * (1.0 + height_range * r[0]))
+        rate = mass_rate * (1.0 if m == 0 else math.exp(rate_sigma * z[m]))
+        out.append(EnsembleMember(int(nx[m]), int(ny[m]), int(nz[m]), nb, top, rate,
+                                  1.0 + wind_range * r[1:], seed))
+    return out
 
 
 def random_problem(nx, ny, nz, nb, seed=0, dx=1000.0, dy=1200.0, dz=150.0, wscale=5.0, stretched=False,

@@ -152,7 +152,7 @@ int fall3d_ads_set_meteo(fall3d_ads_ctx *ctx, const void *u1, const void *u2, co
  * on the device (only nonzero entries are injected). NULL clears it. */
 int fall3d_ads_set_source(fall3d_ads_ctx *ctx, const void *s);
 
-/* Concentration (with ghosts) and ground accumulation; acum may be NULL. */
+/* Concentration (with ghosts) and ground accumulation; NULL sets zero. */
 int fall3d_ads_set_state(fall3d_ads_ctx *ctx, const void *c, const void *acum);
 int fall3d_ads_get_state(fall3d_ads_ctx *ctx, void *c, void *acum);
 
@@ -196,6 +196,96 @@ int fall3d_ads_get_kernel_time(fall3d_ads_ctx *ctx, int kernel, double *total_ms
 /* Algorithmic HBM bytes one launch of sweep `dir` moves (state + coefficient
  * arrays it must read once), for roofline accounting. */
 int fall3d_ads_sweep_bytes(fall3d_ads_ctx *ctx, int dir, double *bytes);
+
+/* ---- 3. Ensemble group --------------------------------------------------
+ * FALL3D runs ensemble members (Sources/mod_Ensemble.F90, task_SetEns) as
+ * independent task_Fall3d copies, one per MPI communicator
+ * (Sources/mod_Parallel.F90:188-196), each with its own grid, perturbed
+ * source and wind, and its own time step. An ensemble here holds one
+ * context per member (sizes may differ; nbins, limiter, time marching and
+ * state mode are common) and advances all of them with one batched launch
+ * per sweep; a run of n steps is replayed from one captured hipGraph.
+ * Members are filled and read through the resident entry points above on
+ * the context returned by fall3d_ads_ensemble_member (borrowed: do not
+ * release it, do not change its stream). No counterpart in the reference. */
+typedef struct fall3d_ads_ensemble fall3d_ads_ensemble;
+
+/* dims: 3*nmembers ints (nx, ny, nz of each member). */
+int fall3d_ads_ensemble_create(fall3d_ads_ensemble **ens, int nmembers, const int *dims, int nbins, int limiter,
+                               int time_marching, int real_kind, int state_mode);
+int fall3d_ads_ensemble_release(fall3d_ads_ensemble *ens);
+const char *fall3d_ads_ensemble_last_error(const fall3d_ads_ensemble *ens);
+int fall3d_ads_ensemble_member(fall3d_ads_ensemble *ens, int m, fall3d_ads_ctx **ctx);
+/* One F3D_time_step of every member; dt[m] and stime[m] per member (stime
+ * may be NULL: 0 for all). Same order of work as fall3d_ads_step. */
+int fall3d_ads_ensemble_step(fall3d_ads_ensemble *ens, const double *dt, const double *stime, int iiter);
+/* nsteps steps from iteration iiter0 (graph replay; asynchronous). */
+int fall3d_ads_ensemble_run(fall3d_ads_ensemble *ens, const double *dt, const double *stime, int iiter0,
+                            int nsteps);
+int fall3d_ads_ensemble_set_graph(fall3d_ads_ensemble *ens, int on);
+int fall3d_ads_ensemble_synchronize(fall3d_ads_ensemble *ens);
+/* Common tile shape: x-sweep cells per lane, y/z bins per workgroup (0 keeps
+ * the current value). get_tiling returns the three values. */
+int fall3d_ads_ensemble_set_tiling(fall3d_ads_ensemble *ens, int x_cells_per_lane, int y_bins_per_group,
+                                   int z_bins_per_group);
+int fall3d_ads_ensemble_get_tiling(fall3d_ads_ensemble *ens, int out[3]);
+/* Autotuner: times every candidate tile shape of each sweep (reps launches
+ * each) on the members as they are, keeps the fastest, restores the state
+ * and accumulators. best_ms (3 doubles, may be NULL) receives the per-sweep
+ * times of the chosen shapes. */
+int fall3d_ads_ensemble_autotune(fall3d_ads_ensemble *ens, const double *dt, const double *stime, int reps,
+                                 double *best_ms);
+
+/* ---- 4. Learned flux correction (coarse grid) ----------------------------
+ * No counterpart in the reference. A per-face network (32 features -> 64 ->
+ * 64 -> 1, GELU, BF16 weights and activations with FP32 accumulation on the
+ * matrix cores, FP32 Adam) corrects the x and y face mass fluxes of a grid
+ * coarsened 2x horizontally. Plain KT is the method and the fallback: a
+ * context runs it unchanged until fall3d_ads_set_learned gives it a network.
+ * Parameters are FP32 in the order W1[32][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3 (fall3d_mlp_nparam() values). Feature rows are 32 bfloat16 bit
+ * patterns (uint16). */
+typedef struct fall3d_mlp fall3d_mlp;
+
+/* capacity: training rows the network keeps on the device (0: none). */
+int fall3d_mlp_create(fall3d_mlp **net, long capacity);
+int fall3d_mlp_release(fall3d_mlp *net);
+const char *fall3d_mlp_last_error(const fall3d_mlp *net);
+int fall3d_mlp_nparam(void);
+/* Set (also resets the Adam moments) / read the FP32 parameters. */
+int fall3d_mlp_set_params(fall3d_mlp *net, const float *params);
+int fall3d_mlp_get_params(fall3d_mlp *net, float *params);
+/* y[r] = network(x[r]), host arrays. */
+int fall3d_mlp_forward(fall3d_mlp *net, const uint16_t *x, long rows, float *y);
+/* Gradient of the mean squared error 1/rows sum (y - t)^2 (host arrays);
+ * sse (may be NULL) receives sum (y - t)^2. */
+int fall3d_mlp_gradient(fall3d_mlp *net, const uint16_t *x, const float *t, long rows, float *grad, double *sse);
+/* Training rows: append host rows, count, read back, clear. */
+int fall3d_mlp_data_append(fall3d_mlp *net, const uint16_t *x, const float *t, long rows);
+int fall3d_mlp_data_count(fall3d_mlp *net, long *rows);
+int fall3d_mlp_data_get(fall3d_mlp *net, long first, long rows, uint16_t *x, float *t);
+int fall3d_mlp_data_clear(fall3d_mlp *net);
+/* steps Adam updates (lr; beta 0.9, 0.999; eps 1e-8) on uniform random
+ * batches of the stored rows; loss (steps floats, may be NULL) receives the
+ * mean squared error of each batch. */
+int fall3d_mlp_train(fall3d_mlp *net, int steps, int batch, float lr, unsigned long long seed, float *loss);
+
+/* x and y face mass flux per unit area of every later sweep, accumulated
+ * (minmod, RK4, FP32 state); the call zeroes the accumulators. get copies
+ * axis 0 (u layout x nbins) or 1 (v layout x nbins) to the host. */
+int fall3d_ads_set_flux_output(fall3d_ads_ctx *ctx, int on);
+int fall3d_ads_get_face_flux(fall3d_ads_ctx *ctx, int axis, float *out);
+/* Use net (borrowed; NULL: plain KT) in every later step: faces whose
+ * stencil maximum is <= smin get no correction. */
+int fall3d_ads_set_learned(fall3d_ads_ctx *ctx, fall3d_mlp *net, double smin);
+/* Interior of coarse := 2x2 horizontal means of fine (fine = 2x coarse). */
+int fall3d_ads_coarse_grain(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse);
+/* One training sample of the whole coarse grid: coarse := coarse-grained
+ * fine; features of the active coarse faces (stencil maximum > smin); one
+ * plain coarse step (dt) and two fine steps (dt/2, advancing fine); targets
+ * from the face fluxes. Appends the rows to net; added receives their count. */
+int fall3d_ads_learned_collect(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse, fall3d_mlp *net, double dt,
+                               double stime, int iiter, double smin, long *added);
 
 #ifdef __cplusplus
 }
