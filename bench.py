@@ -38,6 +38,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 PMC summary (FETCH/WRITE KiB per dispatch) for the traffic field")
+    ap.add_argument("--workload", default="operational", choices=["operational", "ensemble", "learned"],
+                    help="operational: the single 360x240x60x10 domain (default, the headline line); "
+                         "ensemble: 64 members of uneven sizes in batched launches; learned: training and "
+                         "evaluation of the coarse-grid flux correction")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="learned workload: linear scale of the grids (1.0 = 512x512x40 training, "
+                         "1024x768x80x20 evaluation)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the FETCH_SIZE / "
                          "WRITE_SIZE counters against a kernel of known traffic) and exit")
@@ -58,6 +66,140 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
 
 
+def bench_ensemble(args):
+    """64 members, nx, ny ~ U{96..256}, nz ~ U{40..60}, 6 bins, perturbed
+    column height / mass rate / wind modes; autotuned tile shapes; 300 steps
+    (args.steps) of every member as one replayed graph of batched launches."""
+    import numpy as np
+    import torch
+
+    from fall3d_amd import ads
+    from fall3d_amd.problem import ensemble_members
+
+    specs = ensemble_members(args.members, nb=6, seed=3)
+    ens = ads.Ensemble([(m.nx, m.ny, m.nz) for m in specs], 6, "minmod", "rk", 8, args.state)
+    dts = []
+    t0 = time.perf_counter()
+    for g, sp in zip(ens.members, specs):
+        p = sp.problem(dtype_c=np.float32)
+        g.set_grid(p.grid)
+        g.set_meteo(p.u1, p.v1, p.w1, p.k1, p.k2, p.k3, p.vs, p.u2, p.v2, p.w2)
+        g.set_source(p.s)
+        g.set_state(None)
+        dts.append(g.time_step(0.0, 2, 0.9))
+        del p
+    setup_s = time.perf_counter() - t0
+    dts = np.array(dts)
+    tiling, ms = ens.autotune(dts, reps=3)
+    ens.run(dts, max(args.warmup, 2), 0)
+    ens.synchronize()
+    torch.cuda.synchronize()
+    steps = args.steps if args.steps != 500 else 300
+    t1 = time.perf_counter()
+    ens.run(dts, steps, max(args.warmup, 2))
+    ens.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    cb = sum(m.nx * m.ny * m.nz * m.nb for m in specs)
+    cells = sum(m.nx * m.ny * m.nz for m in specs)
+    alg = [cells * (6 * 8 + 12), cells * (6 * 8 + 12), cells * (6 * 8 + 12) + cells * 6 * 4]
+    d = int(np.argmax(ms))
+    ach = alg[d] / (ms[d] / 1e3) / 1e9
+    mass = [g.budget() for g in ens.members]
+    closure = max(abs(b["mass_cells"] + float(np.sum(b["ledger"])) - b["injected"]) / max(b["injected"], 1e-300)
+                  for b in mass)
+    print(json.dumps({
+        "metric": "Gcell*bin updates/s (ADS KT step, ensemble batch)",
+        "value": round(cb * steps / el / 1e9, 3), "unit": "Gcell*bin/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(args.warmup, 2), "ms_per_step": round(el / steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if args.state == "fp32" else "f32x2",
+        "data": "synthetic (perturbed operational members)",
+        "config": {"workload": f"ensemble {args.members} members, nx,ny U(96..256), nz U(40..60), 6 bins",
+                   "cell_bins_per_step": cb, "autotuned_tiling": list(tiling),
+                   "sweep_ms": [round(v, 4) for v in ms], "setup_s": round(setup_s, 1),
+                   "mass_closure_max": closure},
+        "roofline": {"bound": "hbm", "kernel": "sweep_" + "xyz"[d], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": alg[d]},
+        "cpu_baseline": None,
+    }))
+    ens.release()
+
+
+def bench_learned(args):
+    """Train the flux correction on fine/coarse runs (512x512x40, 4 bins, 32
+    seeds at scale 1) and compare plain coarse, learned coarse and plain fine
+    (1024x768x80, 20 bins, 3 line sources at scale 1) against the fine run of
+    the compensated state; throughput in fine-equivalent cell*bin/s."""
+    import numpy as np
+    import torch
+
+    from fall3d_amd import ads, learned
+    from fall3d_amd.problem import config1_wind_problem
+
+    sc = args.scale
+
+    def even(v):
+        return max(8, int(round(v * sc / 8)) * 8)
+
+    tr = (even(512), even(512), max(8, int(40 * sc)), 4)
+    kw_tr = {"dx": 5000.0, "dz": 500.0, "nsources": 1, "puff": True}
+    net = ads.Network(capacity=int(60_000_000 * min(1.0, sc * sc) + 1_000_000))
+    t0 = time.perf_counter()
+    rows = learned.collect(net, range(int(max(4, 32 * min(1.0, sc)))), tr, nwarm=40, nsamples=4, gap=6, **kw_tr)
+    t1 = time.perf_counter()
+    loss = net.train(2000, batch=65536, lr=1e-3, seed=1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ev = (even(1024), even(768), max(8, int(80 * sc)), 20)
+    kw_ev = {"dx": 5000.0, "dz": 250.0, "nsources": 3, "puff": False, "vmin": 0.005, "vmax": 5.0}
+    nsteps = max(10, int(args.steps if args.steps != 500 else 500))  # coarse steps (fine: twice as many)
+    res = learned.evaluate(net, 1234, ev, nsteps, truth_state="fp32x2", **kw_ev)
+    # throughput: plain fine vs learned coarse, same simulated time
+    nx, ny, nz, nb = ev
+    pf = config1_wind_problem(nx, ny, nz, nb, 5000.0, 250.0, 1234, nsources=3, puff=False, vmin=0.005, vmax=5.0)
+    gf = ads.ADS(nx, ny, nz, nb, "minmod", "rk", 8, "fp32")
+    gf.set_problem(pf)
+    pc = config1_wind_problem(nx, ny, nz, nb, 5000.0, 250.0, 1234, coarsen=2, nsources=3, puff=False,
+                              vmin=0.005, vmax=5.0)
+    del pf
+    gc = ads.ADS(pc.nx, pc.ny, pc.nz, nb, "minmod", "rk", 8, "fp32")
+    gc.set_problem(pc)
+    dt = min(gc.time_step(0.0, 2, 0.9), 2.0 * gf.time_step(0.0, 2, 0.9))
+    gc.set_learned(net, 1e-12)
+    k = 20
+    for g, d, n in ((gf, 0.5 * dt, 2 * k), (gc, dt, k)):
+        g.run(d, 4, 0)
+        g.synchronize()
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    gf.run(0.5 * dt, 2 * k, 4)
+    gf.synchronize()
+    tb = time.perf_counter()
+    gc.run(dt, k, 4)
+    gc.synchronize()
+    tc = time.perf_counter()
+    fine_cb = nx * ny * nz * nb * 2 * k
+    thr_fine = fine_cb / (tb - ta) / 1e9
+    thr_learned = fine_cb / (tc - tb) / 1e9
+    print(json.dumps({
+        "metric": "fine-equivalent Gcell*bin/s of the learned coarse step; deposit rel L1 vs fine",
+        "value": round(thr_learned, 3), "unit": "Gcell*bin/s", "n_gpus": 1, "steps": k, "warmup": 4,
+        "ms_per_step": round((tc - tb) / k * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (bf16 network)", "data": "synthetic (config-1 wind, line sources)",
+        "config": {"workload": f"learned coarse {ev[0] // 2}x{ev[1] // 2}x{ev[2]}x{ev[3]} vs fine "
+                               f"{ev[0]}x{ev[1]}x{ev[2]}x{ev[3]}",
+                   "training_grid": list(tr), "training_rows": rows, "collect_s": round(t1 - t0, 1),
+                   "train_s": round(t2 - t1, 2), "loss_first": float(loss[:10].mean()),
+                   "loss_last": float(loss[-50:].mean()), "eval_coarse_steps": nsteps, "errors": res,
+                   "fine_plain_gcbs": round(thr_fine, 3), "speedup_vs_fine": round(thr_learned / thr_fine, 3)},
+        "roofline": None, "cpu_baseline": None,
+    }))
+    gf.release()
+    gc.release()
+    net.release()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -66,6 +208,12 @@ def main():
         sys.exit(2)
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "ensemble":
+        bench_ensemble(args)
+        return
+    if args.workload == "learned":
+        bench_learned(args)
         return
     import numpy as np
     import torch

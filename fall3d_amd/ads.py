@@ -56,6 +56,7 @@ EXPORTS = (
     "fall3d_mlp_create", "fall3d_mlp_release", "fall3d_mlp_last_error", "fall3d_mlp_nparam",
     "fall3d_mlp_set_params", "fall3d_mlp_get_params", "fall3d_mlp_forward", "fall3d_mlp_gradient",
     "fall3d_mlp_data_append", "fall3d_mlp_data_count", "fall3d_mlp_data_get", "fall3d_mlp_data_clear",
+    "fall3d_mlp_data_refs",
     "fall3d_mlp_train", "fall3d_ads_set_flux_output", "fall3d_ads_get_face_flux", "fall3d_ads_set_learned",
     "fall3d_ads_coarse_grain", "fall3d_ads_learned_collect",
 )
@@ -133,6 +134,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_mlp_data_count": ([_vp, ctypes.POINTER(ctypes.c_long)], _int),
         "fall3d_mlp_data_get": ([_vp, ctypes.c_long, ctypes.c_long, _vp, _vp], _int),
         "fall3d_mlp_data_clear": ([_vp], _int),
+        "fall3d_mlp_data_refs": ([_vp, ctypes.c_long, ctypes.c_long, _vp, _vp], _int),
         "fall3d_mlp_train": ([_vp, _int, _int, ctypes.c_float, ctypes.c_ulonglong, _vp], _int),
         "fall3d_ads_set_flux_output": ([_vp, _int], _int),
         "fall3d_ads_get_face_flux": ([_vp, _int, _vp], _int),
@@ -529,7 +531,9 @@ class Network:
     """The per-face flux-correction network (BF16 MFMA kernels, FP32 Adam) and
     its training rows on the device."""
 
-    def __init__(self, capacity=0, params=None):
+    def __init__(self, capacity=0, params=None, seed=0):
+        """capacity: training rows kept on the device; params: FP32 parameter
+        vector, or None for a He-normal initialisation from seed."""
         L = load_library()
         h = _vp()
         rc = L.fall3d_mlp_create(ctypes.byref(h), int(capacity))
@@ -540,8 +544,17 @@ class Network:
             self._h = None
             raise Fall3dAdsError(f"fall3d_mlp_create failed ({rc}): {msg}")
         self.nparam = L.fall3d_mlp_nparam()
-        if params is not None:
-            self.set_params(params)
+        self.set_params(self.initial_params(seed) if params is None else params)
+
+    @staticmethod
+    def initial_params(seed=0):
+        """He-normal weights (std sqrt(2/fan_in); last layer sqrt(1/64)), zero biases,
+        in the order W1[32][64], b1[64], W2[64][64], b2[64], w3[64], b3."""
+        rng = np.random.default_rng(seed)
+        parts = [rng.standard_normal(32 * 64) * np.sqrt(2.0 / 32), np.zeros(64),
+                 rng.standard_normal(64 * 64) * np.sqrt(2.0 / 64), np.zeros(64),
+                 rng.standard_normal(64) * np.sqrt(1.0 / 64), np.zeros(1)]
+        return np.concatenate(parts).astype(np.float32)
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -599,6 +612,15 @@ class Network:
         self._chk(load_library().fall3d_mlp_data_get(self._h, int(first), int(n), x.ctypes.data, t.ctypes.data),
                   "mlp_data_get")
         return x, t
+
+    def refs(self, first=0, n=None):
+        """(face refs (n, 4): axis | bin << 1, f, cross index, k; sgn S dxp (n,)) of collected rows."""
+        n = self.count() - first if n is None else n
+        ref = np.zeros((n, 4), dtype=np.int32)
+        den = np.zeros(n, dtype=np.float32)
+        self._chk(load_library().fall3d_mlp_data_refs(self._h, int(first), int(n), ref.ctypes.data, den.ctypes.data),
+                  "mlp_data_refs")
+        return ref, den
 
     def clear(self):
         self._chk(load_library().fall3d_mlp_data_clear(self._h), "mlp_data_clear")

@@ -13,7 +13,7 @@
 // normal flow and the tangential flow are non-negative, divided by its
 // largest magnitude S; then the Courant numbers |u_n| dt/dxp, |u_t| dt/dx_t,
 // w dt/dz and v_s dt/dz; one zero pad. The network output y gives
-//   dPhi = sgn_n * S * dxp * y   (mass per unit face area over the step),
+//   dPhi = sgn_n * S * dxp * 0.01 y   (mass per unit face area over the step),
 // so the correction scales with the field (the transport is linear in c).
 //
 // Training pairs come from the solver itself: a fine run and a coarse run
@@ -159,6 +159,11 @@ __device__ __forceinline__ float stencil(const FaceArgs &a, const FaceGeo &g, in
   return a.c[cix(a, i, j, kk, b)];
 }
 
+constexpr float kSmallest = 1e-30f;
+// the network predicts kOutScale^-1 times the normalised correction (the
+// corrections are ~1e-2 of S dxp; O(1) outputs train better)
+constexpr float kOutScale = 0.01f;
+
 // Lane (group g = l/16, column l%16) builds features 8g..8g+7 of its face as
 // the B operand of the first layer; returns S (same in all groups).
 __device__ __forceinline__ bf16x8 face_features(const FaceArgs &a, const FaceGeo &geo, bool ok, int k, int b,
@@ -188,11 +193,12 @@ __device__ __forceinline__ bf16x8 face_features(const FaceArgs &a, const FaceGeo
   }
   mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
   mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-  S = mx;
-  const float inv = (mx > 0.0f) ? 1.0f / mx : 0.0f;
+  // stencils below kSmallest (denormal range: 1/S would overflow) are
+  // treated as empty: features zero, face inactive
+  S = (mx > kSmallest) ? mx : 0.0f;
   bf16x8 xb;
 #pragma unroll
-  for (int s = 0; s < 8; ++s) xb[s] = (__bf16)((8 * g + s < 27) ? val[s] * inv : val[s]);
+  for (int s = 0; s < 8; ++s) xb[s] = (__bf16)((8 * g + s < 27) ? (S > 0.0f ? val[s] / S : 0.0f) : val[s]);
   return xb;
 }
 
@@ -264,7 +270,7 @@ __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedW
       tile_forward(w, w1t, w2t, xb, ta);
       const float y = ta.y;
       if (act) {
-        float F = S * geo.dxp * y;  // canonical direction (upwind -> downwind)
+        float F = S * geo.dxp * kOutScale * y;  // canonical direction (upwind -> downwind)
         // positivity: a face may move at most a quarter of its donor cell
         const int ax = a.axis;
         const int iD = (geo.sgn > 0.0f) ? geo.iR : geo.iL, jD = (geo.sgn > 0.0f) ? geo.jR : geo.jL;
@@ -342,7 +348,7 @@ __global__ void __launch_bounds__(256) k_collect_features(FaceArgs a, uint16_t *
 }
 
 // targets of rows r0..r1: (fine flux averaged over the two fine faces of
-// the coarse face - coarse flux) / den
+// the coarse face - coarse flux) / (kOutScale den)
 __global__ void k_collect_targets(const int4 *__restrict__ ref, const float *__restrict__ den, long r0, long r1,
                                   const float *__restrict__ cfx, const float *__restrict__ cfy,
                                   const float *__restrict__ ffx, const float *__restrict__ ffy, int nx, int ny,
@@ -371,7 +377,7 @@ __global__ void k_collect_targets(const int4 *__restrict__ ref, const float *__r
     }
   }
   const float dn = den[r];
-  tgt[r] = (dn != 0.0f) ? (0.5f * pf - pc) / dn : 0.0f;
+  tgt[r] = (dn != 0.0f) ? (0.5f * pf - pc) / (kOutScale * dn) : 0.0f;
 }
 
 // coarse(I, J, k, b) = mean of the 2x2 fine cells (interior only)
@@ -658,6 +664,14 @@ int fall3d_mlp_data_get(fall3d_mlp *n, long r0, long rows, uint16_t *x, float *t
   if (!n || r0 < 0 || rows < 0 || r0 + rows > n->count) return FALL3D_ADS_EINVAL;
   if (x) NHIPCHK(n, hipMemcpyAsync(x, n->x + r0 * 32, sizeof(uint16_t) * 32 * rows, hipMemcpyDeviceToHost, n->stream));
   if (t) NHIPCHK(n, hipMemcpyAsync(t, n->t + r0, sizeof(float) * rows, hipMemcpyDeviceToHost, n->stream));
+  NHIPCHK(n, hipStreamSynchronize(n->stream));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_mlp_data_refs(fall3d_mlp *n, long r0, long rows, int32_t *ref, float *den) {
+  if (!n || r0 < 0 || rows < 0 || r0 + rows > n->count) return FALL3D_ADS_EINVAL;
+  if (ref) NHIPCHK(n, hipMemcpyAsync(ref, n->ref + r0, sizeof(int4) * rows, hipMemcpyDeviceToHost, n->stream));
+  if (den) NHIPCHK(n, hipMemcpyAsync(den, n->den + r0, sizeof(float) * rows, hipMemcpyDeviceToHost, n->stream));
   NHIPCHK(n, hipStreamSynchronize(n->stream));
   return FALL3D_ADS_OK;
 }

@@ -272,6 +272,66 @@ def operational_problem(nx=360, ny=240, nz=60, nb=10, dx=5000.0, dy=5000.0, dz=3
                    k1, k2, k3, vs, c, s, name=f"oper{nx}x{ny}x{nz}x{nb}s{seed}", meta=meta)
 
 
+def config1_wind_problem(nx, ny, nz, nb, dx=5000.0, dz=500.0, seed=0, coarsen=1, vmin=0.05, vmax=2.0,
+                         kh=100.0, kv=10.0, nsources=1, column_top=15000.0, mass_rate=1.0e6, puff=True,
+                         sigma_cells=4.0, peak=1.0, zc=8000.0) -> Problem:
+    """The puff problem's wind generator over open boundaries, with seeded
+    direction and sources; fine/coarse pairs for the learned correction.
+
+    Wind: the speed profile u = 10 + 5 sin(2 pi z/H), v = 3 (the puff
+    problem's), rotated by a seeded angle; w = 0; boundary faces keep their
+    wind (free-flow boundaries). v_s log-spaced vmin..vmax over the bins.
+    Sources: nsources line sources (0..column_top, mass_rate split evenly over
+    bins and column cells) at seeded cells of the middle half; puff=True adds
+    the Gaussian puff (sigma_cells fine cells) at a seeded centre.
+    coarsen=2 builds the same physical problem on the grid coarsened 2x
+    horizontally (nx, ny are the fine sizes): winds at the coarse faces, each
+    line source in the coarse cell that holds its fine cell, the initial field
+    as 2x2 means of the fine one.
+    """
+    rng = np.random.default_rng(seed)
+    theta = rng.uniform(0.0, 2.0 * math.pi)
+    fx = [(int(rng.integers(nx // 4, 3 * nx // 4)) + 1, int(rng.integers(ny // 4, 3 * ny // 4)) + 1)
+          for _ in range(nsources)]
+    pc = (rng.uniform(0.3, 0.7) * nx * dx, rng.uniform(0.3, 0.7) * ny * dx)
+    assert nx % coarsen == 0 and ny % coarsen == 0
+    cx, cy, d = nx // coarsen, ny // coarsen, dx * coarsen
+    H = nz * dz
+    g = uniform_grid(cx, cy, nz, d, d, dz)
+    zk = (np.arange(1, nz + 1) - 0.5) * dz
+    sp = 10.0 + 5.0 * np.sin(2.0 * math.pi * zk / H)
+    ct, st = math.cos(theta), math.sin(theta)
+    u = _F((cx + 3, cy, nz))
+    u[...] = (sp * ct - 3.0 * st)[None, None, :]
+    v = _F((cy + 3, cx, nz))
+    v[...] = (sp * st + 3.0 * ct)[None, None, :]
+    w = _F((nz + 3, cx, cy))
+    k1, k2, k3 = constant_diffusion(cx, cy, nz, kh, kv)
+    vset = vmin * (vmax / vmin) ** (np.arange(nb) / max(nb - 1, 1)) if nb > 1 else np.array([0.5])
+    vs = _F((cx, cy, nz + 3, nb))
+    vs[...] = vset[None, None, None, :]
+    s = _F((cx, cy, nz, nb))
+    for (i, j) in fx:
+        ks = [k for k in range(1, nz + 1) if zk[k - 1] <= column_top]
+        ic, jc = (i - 1) // coarsen + 1, (j - 1) // coarsen + 1
+        s[ic - 1, jc - 1, [k - 1 for k in ks], :] += mass_rate / (nb * len(ks))
+    c = _F((cx + 4, cy + 4, nz + 4, nb))
+    if puff:
+        xf = (np.arange(1, nx + 1) - 0.5) * dx
+        sx = sigma_cells * dx
+        gx = np.exp(-0.5 * ((xf - pc[0]) / sx) ** 2)
+        gy = np.exp(-0.5 * (((np.arange(1, ny + 1) - 0.5) * dx - pc[1]) / sx) ** 2)
+        gz = np.exp(-0.5 * ((zk - zc) / (sigma_cells * dz)) ** 2)
+        if coarsen > 1:
+            gx = gx.reshape(cx, coarsen).mean(1)
+            gy = gy.reshape(cy, coarsen).mean(1)
+        for b in range(nb):
+            c[2:cx + 2, 2:cy + 2, 2:nz + 2, b] = peak * gx[:, None, None] * gy[None, :, None] * gz[None, None, :]
+    return Problem(cx, cy, nz, nb, g, u, u.copy(order="F"), v, v.copy(order="F"), w, w.copy(order="F"),
+                   k1, k2, k3, vs, c, s, name=f"c1wind{cx}x{cy}x{nz}x{nb}s{seed}",
+                   meta={"dx": d, "dz": dz, "seed": seed, "theta": theta, "sources": fx, "vset": vset.tolist()})
+
+
 def latin_hypercube(n, dims, rng):
     """n samples in [-1, 1]^dims with exactly one sample per 1/n stratum along
     every dimension (the reference draws its ensemble perturbations by Latin

@@ -265,6 +265,9 @@ int fall3d_mlp_data_append(fall3d_mlp *net, const uint16_t *x, const float *t, l
 int fall3d_mlp_data_count(fall3d_mlp *net, long *rows);
 int fall3d_mlp_data_get(fall3d_mlp *net, long first, long rows, uint16_t *x, float *t);
 int fall3d_mlp_data_clear(fall3d_mlp *net);
+/* Rows collected by fall3d_ads_learned_collect: face of each row (4 int32:
+ * axis | bin << 1, face index f, cross index, level k) and sgn S dxp. */
+int fall3d_mlp_data_refs(fall3d_mlp *net, long first, long rows, int32_t *ref, float *den);
 /* steps Adam updates (lr; beta 0.9, 0.999; eps 1e-8) on uniform random
  * batches of the stored rows; loss (steps floats, may be NULL) receives the
  * mean squared error of each batch. */
