@@ -148,13 +148,17 @@ def bench_learned(args):
     t0 = time.perf_counter()
     rows = learned.collect(net, range(int(max(4, 32 * min(1.0, sc)))), tr, nwarm=40, nsamples=4, gap=6, **kw_tr)
     t1 = time.perf_counter()
+    print(f"bench: {rows} training rows in {t1 - t0:.1f} s", file=sys.stderr, flush=True)
     loss = net.train(2000, batch=65536, lr=1e-3, seed=1)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    print(f"bench: trained in {t2 - t1:.1f} s, loss {loss[:10].mean():.4e} -> {loss[-50:].mean():.4e}",
+          file=sys.stderr, flush=True)
     ev = (even(1024), even(768), max(8, int(80 * sc)), 20)
     kw_ev = {"dx": 5000.0, "dz": 250.0, "nsources": 3, "puff": False, "vmin": 0.005, "vmax": 5.0}
     nsteps = max(10, int(args.steps if args.steps != 500 else 500))  # coarse steps (fine: twice as many)
     res = learned.evaluate(net, 1234, ev, nsteps, truth_state="fp32x2", **kw_ev)
+    print(f"bench: evaluated {res}", file=sys.stderr, flush=True)
     # throughput: plain fine vs learned coarse, same simulated time
     nx, ny, nz, nb = ev
     pf = config1_wind_problem(nx, ny, nz, nb, 5000.0, 250.0, 1234, nsources=3, puff=False, vmin=0.005, vmax=5.0)
@@ -225,6 +229,7 @@ def main():
     g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, args.state)
     g.set_problem(p)
     dt = g.time_step(0.0, 2, 0.9)
+    tiling, tune_ms = g.autotune(dt, 0.0, 3)
     cells = p.nx * p.ny * p.nz
     cell_bins = cells * p.nb
 
@@ -303,6 +308,8 @@ def main():
             "dt_s": dt,
             "step_hbm_roofline_frac": round(step_alg_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
             "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in ktimes.items()},
+            "autotuned_tiling": {"x_cells_per_lane": tiling[0], "y_bins_per_group": tiling[1],
+                                 "z_bins_per_group": tiling[2]},
             "mass_closure": abs(b["mass_cells"] + float(np.sum(b["ledger"])) - b["injected"]) / max(b["injected"],
                                                                                                      1e-300),
         },

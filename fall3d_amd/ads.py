@@ -48,7 +48,7 @@ EXPORTS = (
     "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
-    "fall3d_ads_set_graph",
+    "fall3d_ads_set_graph", "fall3d_ads_autotune",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
     "fall3d_ads_ensemble_set_graph", "fall3d_ads_ensemble_synchronize", "fall3d_ads_ensemble_set_tiling",
@@ -110,6 +110,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_get_kernel_time": ([_vp, _int, _dp, ctypes.POINTER(ctypes.c_longlong)], _int),
         "fall3d_ads_sweep_bytes": ([_vp, _int, _dp], _int),
         "fall3d_ads_set_graph": ([_vp, _int], _int),
+        "fall3d_ads_autotune": ([_vp, _dbl, _dbl, _int, _vp, _vp], _int),
         "fall3d_ads_ensemble_create": ([ctypes.POINTER(_vp), _int, ctypes.POINTER(_int), _int, _int, _int, _int,
                                         _int], _int),
         "fall3d_ads_ensemble_release": ([_vp], _int),
@@ -401,6 +402,14 @@ class ADS:
     def set_tiling(self, axis, cells_per_thread=0, columns=0, bins_per_group=0):
         self._chk(load_library().fall3d_ads_set_tiling(self._h, int(axis), int(cells_per_thread), int(columns),
                                                        int(bins_per_group)), "set_tiling")
+
+    def autotune(self, dt, stime=0.0, reps=3):
+        """Time the candidate tile shapes of each sweep and keep the fastest
+        (state restored); returns ((x cells per lane, y bins, z bins), per-sweep ms)."""
+        t = (_int * 3)()
+        ms = (ctypes.c_double * 3)()
+        self._chk(load_library().fall3d_ads_autotune(self._h, float(dt), float(stime), int(reps), t, ms), "autotune")
+        return tuple(t), tuple(ms)
 
     def set_graph(self, on=True):
         self._chk(load_library().fall3d_ads_set_graph(self._h, 1 if on else 0), "set_graph")

@@ -993,6 +993,106 @@ int fall3d_ads_synchronize(fall3d_ads_ctx *c) {
   return FALL3D_ADS_OK;
 }
 
+// Autotuner of the tile shapes: every candidate of each sweep is timed
+// (reps launches between HIP events on the context stream, after one
+// warm-up launch) on the context's own fields; the fastest is kept. The
+// sweeps change the state, so state, deposit and accumulators are saved
+// before and restored after.
+int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, int tiling[3], double best_ms[3]) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  int rc;
+  if ((rc = need_grid(c))) return rc;
+  if (!c->meteo_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_meteo has not been called");
+  if (reps < 1) reps = 3;
+  if ((rc = ensure_dt(c, dt))) return rc;
+  // save
+  void *src[4] = {c->chi, c->comp ? c->clo : nullptr, c->acum, c->acc};
+  const size_t bytes[4] = {c->ncell_c * sizeof(float), c->ncell_c * sizeof(float), c->nacum * sizeof(double),
+                           (size_t)ACC_LEN * sizeof(double)};
+  void *save[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int i = 0; i < 4; ++i) {
+    if (!src[i]) continue;
+    HIPCHK(c, hipMalloc(&save[i], bytes[i]));
+    HIPCHK(c, hipMemcpyAsync(save[i], src[i], bytes[i], hipMemcpyDeviceToDevice, c->stream));
+  }
+  hipEvent_t ea = nullptr, eb = nullptr;
+  HIPCHK(c, hipEventCreate(&ea));
+  HIPCHK(c, hipEventCreate(&eb));
+  const bool prof = c->prof;
+  c->prof = false;
+  const int oc = c->user_cpt[0], ob[3] = {c->user_bpg[0], c->user_bpg[1], c->user_bpg[2]};
+  double best[3] = {1e30, 1e30, 1e30};
+  int pick[3] = {0, 0, 0};
+  for (int d = 0; d < 3 && !rc; ++d) {
+    std::vector<int> cand;
+    if (d == 0) {
+      for (int i = 0; i < kSweepCptCount; ++i)
+        if ((c->nx + kSweepCpt[i] - 1) / kSweepCpt[i] <= 64) cand.push_back(kSweepCpt[i]);
+    } else {
+      for (int g = 1; g <= std::min(c->nb, 8); ++g) {
+        const int groups = (c->nb + g - 1) / g;
+        if ((c->nb + groups - 1) / groups == g) cand.push_back(g);
+      }
+    }
+    for (int v : cand) {
+      if (d == 0)
+        c->user_cpt[0] = v;
+      else
+        c->user_bpg[d] = v;
+      if ((rc = refresh_tiling(c))) break;
+      const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
+      const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
+      const float *kd = d == 0 ? c->k1 : (d == 1 ? c->k2 : c->k3);
+      SweepOut so;
+      if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so))) break;
+      (void)hipEventRecord(ea, c->stream);
+      for (int r = 0; r < reps && !rc; ++r) rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so);
+      (void)hipEventRecord(eb, c->stream);
+      if (rc) break;
+      (void)hipEventSynchronize(eb);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ea, eb);
+      if (ms / reps < best[d]) {
+        best[d] = ms / reps;
+        pick[d] = v;
+      }
+    }
+    if (d == 0)
+      c->user_cpt[0] = pick[0] ? pick[0] : oc;
+    else
+      c->user_bpg[d] = pick[d] ? pick[d] : ob[d];
+  }
+  c->prof = prof;
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  // restore
+  for (int i = 0; i < 4; ++i) {
+    if (!save[i]) continue;
+    if (!rc) {
+      const hipError_t e = hipMemcpyAsync(src[i], save[i], bytes[i], hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) rc = fail(c, FALL3D_ADS_EHIP, "autotune restore: %s", hipGetErrorString(e));
+    }
+  }
+  (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 4; ++i)
+    if (save[i]) (void)hipFree(save[i]);
+  if (rc) {
+    c->user_cpt[0] = oc;
+    for (int d = 0; d < 3; ++d) c->user_bpg[d] = ob[d];
+    (void)refresh_tiling(c);
+    return rc;
+  }
+  if ((rc = refresh_tiling(c))) return rc;
+  if (tiling) {
+    tiling[0] = c->tl[0].cpt;
+    tiling[1] = c->tl[1].bpg;
+    tiling[2] = c->tl[2].bpg;
+  }
+  if (best_ms)
+    for (int d = 0; d < 3; ++d) best_ms[d] = best[d];
+  return FALL3D_ADS_OK;
+}
+
 int fall3d_ads_set_tiling(fall3d_ads_ctx *c, int dir, int cells_per_thread, int columns, int bins_per_group) {
   if (!c || dir < 0 || dir > 2) return FALL3D_ADS_EINVAL;
   if (cells_per_thread && !sweep_cpt_supported(cells_per_thread))

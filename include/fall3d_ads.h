@@ -187,6 +187,11 @@ int fall3d_ads_synchronize(fall3d_ads_ctx *ctx);
 /* Tuning: cells per thread (0 = automatic) and x-columns per workgroup for
  * the sweep of axis dir (0 x, 1 y, 2 z); bins per workgroup. */
 int fall3d_ads_set_tiling(fall3d_ads_ctx *ctx, int dir, int cells_per_thread, int columns, int bins_per_group);
+/* Autotuner: times every candidate tile shape of each sweep (x: cells per
+ * lane; y, z: bins per workgroup) on the context's fields, keeps the
+ * fastest and restores state, deposit and accumulators. tiling (3 ints) and
+ * best_ms (3 doubles) may be NULL. */
+int fall3d_ads_autotune(fall3d_ads_ctx *ctx, double dt, double stime, int reps, int tiling[3], double best_ms[3]);
 
 /* Per-kernel timing with HIP events on the context stream.
  * kernel ids: 0 source, 1 sweep x, 2 sweep y, 3 sweep z, 4 reduce. */

@@ -132,3 +132,28 @@ def test_ensemble_autotune_restores_state_and_keeps_results():
         c_s, _, _ = _alone(sp, tiling, "fp32", "rk", dts[m], 7)
         assert np.array_equal(ens.members[m].get_state()[0], c_s), m
     ens.release()
+
+
+def test_single_domain_autotune_restores_state():
+    from fall3d_amd.problem import operational_problem
+    p = operational_problem(70, 50, 16, 5, seed=4)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    g.set_problem(p)
+    dt = g.time_step(0.0, 2, 0.9)
+    g.run(dt, 3, 0)
+    c0, a0 = g.get_state()
+    b0 = g.budget()
+    tiling, ms = g.autotune(dt, 0.0, 2)
+    print(f"tiling {tiling} ms {ms}", flush=True)
+    c1, a1 = g.get_state()
+    assert np.array_equal(c0, c1) and np.array_equal(a0, a1)
+    assert g.budget()["mass_cells"] == b0["mass_cells"]
+    g.run(dt, 4, 3)
+    c_t = g.get_state()[0]
+    g.release()
+    # the concentration does not depend on the tile shape
+    h = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    h.set_problem(p)
+    h.run(dt, 7, 0)
+    assert np.array_equal(h.get_state()[0], c_t)
+    h.release()

@@ -69,5 +69,6 @@ def test_trained_correction_beats_plain_coarse_kt():
     assert loss[-20:].mean() < loss[:5].mean()
     res = learned.evaluate(net, 100, SIZE, 24, **KW)
     print(f"held-out seed: {res}", flush=True)
-    assert res["learned"]["load_l1"] < res["plain"]["load_l1"]
+    assert res["learned"]["load_l1"] < 0.8 * res["plain"]["load_l1"]
+    assert res["learned"]["deposit_l1"] < res["plain"]["deposit_l1"]
     net.release()
