@@ -293,52 +293,55 @@ hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double
 
 // One wave per (array, slot): sums partials[b*4+slot] over b in a fixed
 // lane-strided order, then a fixed shuffle tree; adds into acc[dst].
-// Sum of p[b*4 + slot] over b < nb by one wave, in a fixed order: lane l
-// keeps 8 running sums over b = l + 64 (8 k + r) (independent loads in
-// flight), combines them r = 0..7, then the shuffle tree.
-__device__ __forceinline__ double wave_slot_sum(const double *__restrict__ p, int nb, int slot) {
-  const int lane = threadIdx.x & 63;
+// Sum of p[b*4 + slot] over b < nb by one 256-thread workgroup, in a fixed
+// order: thread t keeps 8 running sums over b = t + 256 (8 k + r) (8
+// independent loads in flight), combines them r = 0..7, then a shuffle tree
+// per wave and the 4 wave sums in order. Valid in thread 0.
+__device__ __forceinline__ double block_slot_sum(const double *__restrict__ p, int nb, int slot) {
+  __shared__ double red[4];
+  const int t = threadIdx.x;
   double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int b0 = 0; b0 < nb; b0 += 512) {
+  for (int b0 = 0; b0 < nb; b0 += 2048) {
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int b = b0 + 64 * r + lane;
+      const int b = b0 + 256 * r + t;
       if (b < nb) s[r] += p[(size_t)b * 4 + slot];
     }
   }
-  double t = s[0];
+  double v = s[0];
 #pragma unroll
-  for (int r = 1; r < 8; ++r) t += s[r];
-  return wsum(t);
+  for (int r = 1; r < 8; ++r) v += s[r];
+  v = wsum(v);
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  return ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-__global__ void k_finalize(double *__restrict__ acc, ReduceJob jobs) {
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (wid >= jobs.count) return;
-  const double s = wave_slot_sum(jobs.part[wid], jobs.nblocks[wid], jobs.slot[wid]);
-  if (lane == 0) acc[jobs.dst[wid]] += s;
+// one workgroup (256 threads) per job
+__global__ void __launch_bounds__(256) k_finalize(double *__restrict__ acc, ReduceJob jobs) {
+  const int j = blockIdx.x;
+  const double s = block_slot_sum(jobs.part[j], jobs.nblocks[j], jobs.slot[j]);
+  if (threadIdx.x == 0) acc[jobs.dst[j]] += s;
 }
 
-// Ensemble form: one wave per job of a device job table.
-__global__ void k_finalize_batch(const BatchJob *__restrict__ jobs, int njobs) {
-  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (j >= njobs) return;
-  const BatchJob job = jobs[j];
-  const double s = wave_slot_sum(job.part, job.nblocks, job.slot);
-  if ((threadIdx.x & 63) == 0) *job.dst += s;
+// Ensemble form: one workgroup per job of a device job table.
+__global__ void __launch_bounds__(256) k_finalize_batch(const BatchJob *__restrict__ jobs, int njobs) {
+  const BatchJob job = jobs[blockIdx.x];
+  const double s = block_slot_sum(job.part, job.nblocks, job.slot);
+  if (threadIdx.x == 0) *job.dst += s;
 }
 
 hipError_t finalize_batch(const BatchJob *jobs, int njobs, hipStream_t st) {
   (void)hipGetLastError();
   if (njobs == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize_batch, dim3((njobs + 3) / 4), dim3(256), 0, st, jobs, njobs);
+  hipLaunchKernelGGL(k_finalize_batch, dim3(njobs), dim3(256), 0, st, jobs, njobs);
   return hipGetLastError();
 }
 
 hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
   if (jobs.count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64 * jobs.count), 0, st, acc, jobs);
+  hipLaunchKernelGGL(k_finalize, dim3(jobs.count), dim3(256), 0, st, acc, jobs);
   return hipGetLastError();
 }
 

@@ -111,6 +111,7 @@ struct fall3d_ads_ctx {
   struct fall3d_mlp *learned = nullptr;  // flux-correction network (borrowed) or nullptr: plain KT
   float *dphi[2] = {nullptr, nullptr};   // its face corrections of the current step
   double learned_smin = 0.0;
+  double learned_gain = 1.0;
 };
 
 namespace f3d {

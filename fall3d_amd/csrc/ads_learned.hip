@@ -72,6 +72,7 @@ struct FaceArgs {
   const float *hdxp[3];  // 0.5/dxp(f-1) at face f
   const float *idxb[3];  // 1/dxb(p) at cell p
   float smin;            // faces with S <= smin are inactive
+  float gain;            // fraction of the predicted correction applied
 };
 
 __device__ __forceinline__ size_t cix(const FaceArgs &a, int i, int j, int k, int b) {
@@ -279,7 +280,7 @@ __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedW
         const float dd = 1.0f / a.idxb[ax][ax == 0 ? iD : jD];
         F = fminf(F, 0.25f * fmaxf(cu, 0.0f) * du);
         F = fmaxf(F, -0.25f * fmaxf(cd, 0.0f) * dd);
-        out = geo.sgn * F;
+        out = a.gain * geo.sgn * F;
       }
     }
     if (ok && l < 16) dphi[face_index(a, f, tr, k, b)] = out;
@@ -463,6 +464,7 @@ FaceArgs face_args(fall3d_ads_ctx *c, int axis, double dt, double stime, double 
     a.idxb[d] = c->idxb[d];
   }
   a.smin = (float)smin;
+  a.gain = 1.0f;
   return a;
 }
 
@@ -503,6 +505,7 @@ namespace host {
 int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
   for (int axis = 0; axis < 2; ++axis) {
     FaceArgs a = face_args(c, axis, dt, stime, c->learned_smin);
+    a.gain = (float)c->learned_gain;
     const long nt = ntiles(c, axis);
     hipLaunchKernelGGL(k_correct_faces, dim3(grid_for(nt)), dim3(256), 0, c->stream, a, c->learned->packed,
                        c->dphi[axis]);
@@ -748,7 +751,7 @@ int fall3d_ads_get_face_flux(fall3d_ads_ctx *c, int axis, float *out) {
   return FALL3D_ADS_OK;
 }
 
-int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin) {
+int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, double gain) {
   if (!c) return FALL3D_ADS_EINVAL;
   if (net) {
     if (c->comp) return fail(c, FALL3D_ADS_EINVAL, "the learned correction runs on the FP32 state only");
@@ -759,6 +762,7 @@ int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin) {
   }
   c->learned = net;
   c->learned_smin = smin;
+  c->learned_gain = gain;
   c->version++;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FALL3D_ADS_OK;

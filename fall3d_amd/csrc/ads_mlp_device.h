@@ -14,15 +14,18 @@ namespace f3d {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
-__device__ __forceinline__ float gelu_f(float x) {
+// tanh-form GELU through the identity 0.5 (1 + tanh u) = 1 / (1 + exp(-2u)):
+// gelu(x) = x sigma(2u), gelu'(x) = sigma + 2 x sigma (1 - sigma) u',
+// u = sqrt(2/pi) (x + 0.044715 x^3). One hardware exp and one reciprocal.
+__device__ __forceinline__ float gelu_sig(float x) {
   const float u = 0.7978845608028654f * fmaf(0.044715f * x, x * x, x);
-  return 0.5f * x * (1.0f + tanhf(u));
+  return __fdividef(1.0f, 1.0f + __expf(-2.0f * u));
 }
+__device__ __forceinline__ float gelu_f(float x) { return x * gelu_sig(x); }
 __device__ __forceinline__ float gelu_df(float x) {
-  const float u = 0.7978845608028654f * fmaf(0.044715f * x, x * x, x);
-  const float t = tanhf(u);
+  const float s = gelu_sig(x);
   const float du = 0.7978845608028654f * fmaf(3.0f * 0.044715f, x * x, 1.0f);
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
+  return fmaf(2.0f * x * s * (1.0f - s), du, s);
 }
 
 // lanes of one wave exchange data through LDS: order this wave's LDS

@@ -74,7 +74,7 @@ def rel_l1(a, ref):
     return float(np.abs(a - ref).sum() / max(np.abs(ref).sum(), 1e-300))
 
 
-def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", **kw):
+def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", gain=1.0, **kw):
     """Run fine (2 nsteps steps), coarse plain and coarse learned (nsteps steps
     each) from the same initial state; compare the coarse column loads and
     ground deposits with the fine ones averaged onto the coarse grid.
@@ -95,7 +95,7 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", *
                                   **{k: v for k, v in kw.items() if k not in ("dx", "dz")})
         g = ads.ADS(pc.nx, pc.ny, pc.nz, pc.nb, "minmod", "rk", 8, "fp32")
         g.set_problem(pc)
-        g.set_learned(learned, smin)
+        g.set_learned(learned, smin, gain)
         g.run(dt, nsteps, 0)
         cc, ac = g.get_state()
         g.release()
