@@ -43,6 +43,9 @@ def parse():
                          "ensemble: 64 members of uneven sizes in batched launches; learned: training and "
                          "evaluation of the coarse-grid flux correction")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--tiling", default="",
+                    help="x_cells_per_lane,y_bins_per_group,z_bins_per_group: fixed tile shapes instead of "
+                         "the autotuner (profiling runs)")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="learned workload: linear scale of the grids (1.0 = 512x512x40 training, "
                          "1024x768x80x20 evaluation)")
@@ -143,7 +146,7 @@ def bench_learned(args):
         return max(8, int(round(v * sc / 8)) * 8)
 
     tr = (even(512), even(512), max(8, int(40 * sc)), 4)
-    kw_tr = {"dx": 5000.0, "dz": 500.0, "nsources": 1, "puff": True}
+    kw_tr = {"dx": 5000.0, "dz": 500.0, "nsources": 2, "puff": True, "vmin": 0.005, "vmax": 5.0}
     net = ads.Network(capacity=int(60_000_000 * min(1.0, sc * sc) + 1_000_000))
     t0 = time.perf_counter()
     rows = learned.collect(net, range(int(max(4, 32 * min(1.0, sc)))), tr, nwarm=40, nsamples=4, gap=6, **kw_tr)
@@ -157,7 +160,8 @@ def bench_learned(args):
     ev = (even(1024), even(768), max(8, int(80 * sc)), 20)
     kw_ev = {"dx": 5000.0, "dz": 250.0, "nsources": 3, "puff": False, "vmin": 0.005, "vmax": 5.0}
     nsteps = max(10, int(args.steps if args.steps != 500 else 500))  # coarse steps (fine: twice as many)
-    res = learned.evaluate(net, 1234, ev, nsteps, truth_state="fp32x2", **kw_ev)
+    res = learned.evaluate(net, 1234, ev, nsteps, smin=1e-10, truth_state="fp32x2", fine_fp32=True,
+                           gains=(1.0, 0.5), **kw_ev)
     print(f"bench: evaluated {res}", file=sys.stderr, flush=True)
     # throughput: plain fine vs learned coarse, same simulated time
     nx, ny, nz, nb = ev
@@ -170,7 +174,7 @@ def bench_learned(args):
     gc = ads.ADS(pc.nx, pc.ny, pc.nz, nb, "minmod", "rk", 8, "fp32")
     gc.set_problem(pc)
     dt = min(gc.time_step(0.0, 2, 0.9), 2.0 * gf.time_step(0.0, 2, 0.9))
-    gc.set_learned(net, 1e-12)
+    gc.set_learned(net, 1e-10)
     k = 20
     for g, d, n in ((gf, 0.5 * dt, 2 * k), (gc, dt, k)):
         g.run(d, 4, 0)
@@ -229,7 +233,14 @@ def main():
     g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, args.state)
     g.set_problem(p)
     dt = g.time_step(0.0, 2, 0.9)
-    tiling, tune_ms = g.autotune(dt, 0.0, 3)
+    if args.tiling:
+        tx, ty, tz = (int(v) for v in args.tiling.split(","))
+        g.set_tiling(0, cells_per_thread=tx)
+        g.set_tiling(1, bins_per_group=ty)
+        g.set_tiling(2, bins_per_group=tz)
+        tiling = (tx, ty, tz)
+    else:
+        tiling, tune_ms = g.autotune(dt, 0.0, 3)
     cells = p.nx * p.ny * p.nz
     cell_bins = cells * p.nb
 
