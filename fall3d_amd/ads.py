@@ -139,7 +139,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_mlp_train": ([_vp, _int, _int, ctypes.c_float, ctypes.c_ulonglong, _vp], _int),
         "fall3d_ads_set_flux_output": ([_vp, _int], _int),
         "fall3d_ads_get_face_flux": ([_vp, _int, _vp], _int),
-        "fall3d_ads_set_learned": ([_vp, _vp, _dbl, _dbl], _int),
+        "fall3d_ads_set_learned": ([_vp, _vp, _dbl, _dbl, _dbl], _int),
         "fall3d_ads_coarse_grain": ([_vp, _vp], _int),
         "fall3d_ads_learned_collect": ([_vp, _vp, _vp, _dbl, _dbl, _int, _dbl, ctypes.POINTER(ctypes.c_long)],
                                        _int),
@@ -436,11 +436,13 @@ class ADS:
         self._chk(load_library().fall3d_ads_get_face_flux(self._h, int(axis), out.ctypes.data), "get_face_flux")
         return out
 
-    def set_learned(self, net=None, smin=0.0, gain=1.0):
-        """Correct the x/y face fluxes of later steps with net (None: plain KT)."""
+    def set_learned(self, net=None, smin=0.0, gain=1.0, flux_limit=0.0):
+        """Correct the x/y face fluxes of later steps with net (None: plain KT).
+        smin < 0: threshold relative to the field maximum; flux_limit > 0: each
+        correction bounded by flux_limit times the plain face flux."""
         self._keep["learned"] = net
         self._chk(load_library().fall3d_ads_set_learned(self._h, net._h if net is not None else None, float(smin),
-                                                        float(gain)), "set_learned")
+                                                        float(gain), float(flux_limit)), "set_learned")
 
     def sweep_bytes(self, axis):
         b = _dbl()

@@ -223,9 +223,12 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
   }
   a.acum = c->acum;
   a.partials = c->part[dir + 1];
-  if (c->fout_on && dir < 2) {
+  if (dir < 2 && (c->fout_on || (c->learned && c->learned_flux_limit > 0.0))) {
+    // face fluxes: accumulated for training rows, overwritten for the
+    // learned step's flux-relative clamp
     a.fout = c->fout[dir];
     a.fscale = (float)(c->tm == 1 ? dt : dt / 6.0);
+    a.fout_add = c->fout_on ? 1 : 0;
   }
 }
 

@@ -112,6 +112,9 @@ struct fall3d_ads_ctx {
   float *dphi[2] = {nullptr, nullptr};   // its face corrections of the current step
   double learned_smin = 0.0;
   double learned_gain = 1.0;
+  double learned_flux_limit = 0.0;       // |dPhi| <= limit * |plain face flux| (0: off)
+  float *lthr = nullptr;                 // device threshold (relative smin) and its partials
+  int lthr_blocks = 0;
 };
 
 namespace f3d {

@@ -49,6 +49,7 @@ struct SweepArgs {
   double *partials;          // [gridDim.x*gridDim.y][4]: ref flux beg/end, ledger beg/end
   float *fout;               // optional face mass flux output (x: u layout, y: v layout, per bin), or nullptr
   float fscale;              // fout += fscale * (G1 + 2 G2 + 2 G3 + G4) (RK4: dt/6; Euler: dt * G)
+  int fout_add;              // 1: accumulate into fout; 0: overwrite (every interior face written)
 };
 
 // One launch over the members of an ensemble: member m owns the workgroups

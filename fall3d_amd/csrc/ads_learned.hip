@@ -72,6 +72,7 @@ struct FaceArgs {
   const float *hdxp[3];  // 0.5/dxp(f-1) at face f
   const float *idxb[3];  // 1/dxb(p) at cell p
   float smin;            // faces with S <= smin are inactive
+  const float *smin_dev; // or, when set, the threshold in device memory
   float gain;            // fraction of the predicted correction applied
 };
 
@@ -257,14 +258,29 @@ __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedW
   const long nwave = (long)gridDim.x * (blockDim.x >> 6);
   bf16x8 w1t[4], w2t[4][2];
   load_weights(w, w1t, w2t);
+  const float smin = a.smin_dev ? *a.smin_dev : a.smin;
   for (long t = wave; t < ntile; t += nwave) {
     int f, tr, k, b;
     const bool ok = tile_face(a, t, col, f, tr, k, b);
+    // cheap activity test before the stencil gather: a tile whose faces
+    // have both cells at or below the threshold gets no correction
+    {
+      bool live = false;
+      if (ok) {
+        const int i0 = (a.axis == 0) ? f - 1 : tr, j0 = (a.axis == 0) ? tr : f - 1;
+        const int i1 = (a.axis == 0) ? f : tr, j1 = (a.axis == 0) ? tr : f;
+        live = fmaxf(fabsf(a.c[cix(a, i0, j0, k, b)]), fabsf(a.c[cix(a, i1, j1, k, b)])) > fmaxf(smin, kSmallest);
+      }
+      if (!__ballot(live)) {
+        if (ok && l < 16) dphi[face_index(a, f, tr, k, b)] = 0.0f;
+        continue;
+      }
+    }
     FaceGeo geo;
     if (ok) geo = face_geo(a, f, tr, k, b);
     float S = 0.0f;
     const bf16x8 xb = face_features(a, geo, ok, k, b, S);
-    const bool act = ok && S > a.smin;
+    const bool act = ok && S > smin;
     float out = 0.0f;
     if (__ballot(act)) {  // uniform: skip the network on tiles without mass
       TileAct ta;
@@ -287,8 +303,20 @@ __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedW
   }
 }
 
-// c -= (dX(i+1) - dX(i)) / dxb_x(i) + (dY(j+1) - dY(j)) / dxb_y(j)
+// correction of one face, limited to lim |plain face flux| (lim <= 0: off)
+__device__ __forceinline__ float limited(const float *__restrict__ d, const float *__restrict__ phi, size_t i,
+                                         float lim) {
+  const float v = d[i];
+  if (lim <= 0.0f) return v;
+  const float m = lim * fabsf(phi[i]);
+  return fminf(fmaxf(v, -m), m);
+}
+
+// c -= (dX(i+1) - dX(i)) / dxb_x(i) + (dY(j+1) - dY(j)) / dxb_y(j); both
+// cells of a face see the same limited value, so the update stays
+// conservative
 __global__ void k_apply_corrections(float *__restrict__ c, const float *__restrict__ dx, const float *__restrict__ dy,
+                                    const float *__restrict__ px, const float *__restrict__ py, float lim,
                                     int nx, int ny, int nz, int nb, const float *__restrict__ idxb_x,
                                     const float *__restrict__ idxb_y) {
   const size_t n = (size_t)nx * ny * nz * nb;
@@ -299,7 +327,8 @@ __global__ void k_apply_corrections(float *__restrict__ c, const float *__restri
     const int b = (int)(q / ((size_t)nx * ny * nz));
     const size_t fx = (size_t)i + (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * ((size_t)(k - 1) + (size_t)nz * b));
     const size_t fy = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * ((size_t)(k - 1) + (size_t)nz * b));
-    const float d = (dx[fx + 1] - dx[fx]) * idxb_x[i] + (dy[fy + nx] - dy[fy]) * idxb_y[j];
+    const float d = (limited(dx, px, fx + 1, lim) - limited(dx, px, fx, lim)) * idxb_x[i] +
+                    (limited(dy, py, fy + nx, lim) - limited(dy, py, fy, lim)) * idxb_y[j];
     const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
     c[ci] -= d;
   }
@@ -379,6 +408,26 @@ __global__ void k_collect_targets(const int4 *__restrict__ ref, const float *__r
   }
   const float dn = den[r];
   tgt[r] = (dn != 0.0f) ? (0.5f * pf - pc) / (kOutScale * dn) : 0.0f;
+}
+
+// largest |c| of a field: per-block partial maxima, then the threshold
+// rel * max in out[0] (maxima do not depend on the order)
+__global__ void k_absmax(const float *__restrict__ c, size_t n, float *__restrict__ part) {
+  __shared__ float red[4];
+  float m = 0.0f;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(c[q]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+__global__ void k_threshold(const float *__restrict__ part, int nb, float rel, float *__restrict__ out) {
+  float m = 0.0f;
+  for (int b = threadIdx.x; b < nb; b += 64) m = fmaxf(m, part[b]);
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if (threadIdx.x == 0) out[0] = rel * m;
 }
 
 // coarse(I, J, k, b) = mean of the 2x2 fine cells (interior only)
@@ -464,6 +513,7 @@ FaceArgs face_args(fall3d_ads_ctx *c, int axis, double dt, double stime, double 
     a.idxb[d] = c->idxb[d];
   }
   a.smin = (float)smin;
+  a.smin_dev = nullptr;
   a.gain = 1.0f;
   return a;
 }
@@ -503,8 +553,17 @@ namespace f3d {
 namespace host {
 
 int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
+  const bool rel = c->learned_smin < 0.0;
+  if (rel) {  // threshold relative to the field's largest value
+    hipLaunchKernelGGL(k_absmax, dim3(c->lthr_blocks), dim3(256), 0, c->stream, c->chi, c->ncell_c, c->lthr + 1);
+    hipLaunchKernelGGL(k_threshold, dim3(1), dim3(64), 0, c->stream, c->lthr + 1, c->lthr_blocks,
+                       (float)(-c->learned_smin), c->lthr);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "learned threshold: %s", hipGetErrorString(e));
+  }
   for (int axis = 0; axis < 2; ++axis) {
-    FaceArgs a = face_args(c, axis, dt, stime, c->learned_smin);
+    FaceArgs a = face_args(c, axis, dt, stime, rel ? 0.0 : c->learned_smin);
+    if (rel) a.smin_dev = c->lthr;
     a.gain = (float)c->learned_gain;
     const long nt = ntiles(c, axis);
     hipLaunchKernelGGL(k_correct_faces, dim3(grid_for(nt)), dim3(256), 0, c->stream, a, c->learned->packed,
@@ -518,8 +577,10 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
 int learned_post_step(fall3d_ads_ctx *c) {
   const size_t n = (size_t)c->nx * c->ny * c->nz * c->nb;
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  const float lim = (float)c->learned_flux_limit;
   hipLaunchKernelGGL(k_apply_corrections, dim3(blocks), dim3(256), 0, c->stream, c->chi, c->dphi[0], c->dphi[1],
-                     c->nx, c->ny, c->nz, c->nb, c->idxb[0], c->idxb[1]);
+                     lim > 0.0f ? c->fout[0] : c->dphi[0], lim > 0.0f ? c->fout[1] : c->dphi[1], lim, c->nx, c->ny,
+                     c->nz, c->nb, c->idxb[0], c->idxb[1]);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "apply corrections: %s", hipGetErrorString(e));
   return FALL3D_ADS_OK;
@@ -531,6 +592,8 @@ void learned_release(fall3d_ads_ctx *c) {
     dfree(c->dphi[a]);
     c->fout[a] = c->dphi[a] = nullptr;
   }
+  dfree(c->lthr);
+  c->lthr = nullptr;
   c->learned = nullptr;
 }
 
@@ -751,18 +814,29 @@ int fall3d_ads_get_face_flux(fall3d_ads_ctx *c, int axis, float *out) {
   return FALL3D_ADS_OK;
 }
 
-int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, double gain) {
+int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, double gain, double flux_limit) {
   if (!c) return FALL3D_ADS_EINVAL;
   if (net) {
     if (c->comp) return fail(c, FALL3D_ADS_EINVAL, "the learned correction runs on the FP32 state only");
+    if (flux_limit > 0.0 && (c->limiter != 1 || c->tm != 2))
+      return fail(c, FALL3D_ADS_EINVAL, "the flux-relative limit needs minmod and RK4");
     for (int a = 0; a < 2; ++a) {
       if (!c->dphi[a]) HIPCHK(c, dalloc(&c->dphi[a], nface(c, a)));
       HIPCHK(c, hipMemsetAsync(c->dphi[a], 0, sizeof(float) * nface(c, a), c->stream));
+      if (flux_limit > 0.0 && !c->fout[a]) {
+        HIPCHK(c, dalloc(&c->fout[a], nface(c, a)));
+        HIPCHK(c, hipMemsetAsync(c->fout[a], 0, sizeof(float) * nface(c, a), c->stream));
+      }
+    }
+    if (!c->lthr) {
+      c->lthr_blocks = 1024;
+      HIPCHK(c, dalloc(&c->lthr, (size_t)c->lthr_blocks + 1));
     }
   }
   c->learned = net;
   c->learned_smin = smin;
   c->learned_gain = gain;
+  c->learned_flux_limit = net ? flux_limit : 0.0;
   c->version++;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FALL3D_ADS_OK;

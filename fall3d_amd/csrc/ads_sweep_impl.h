@@ -432,9 +432,10 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
         if constexpr (FOUT) {  // face mass flux per unit area: faces q (1..n) and n+1
           const uint32_t fo = fpos(q), fo1 = fpos(q + 1);
           const uint32_t vf = (vo != kOOB) ? vv : kOOB;  // same predicate, face-array lane offset
-          bstore(rf[j], vf, fo, fmaf(a.fscale, gq, bload(rf[j], vv, fo)));
+          const bool add = a.fout_add != 0;
+          bstore(rf[j], vf, fo, fmaf(a.fscale, gq, add ? bload(rf[j], vv, fo) : 0.0f));
           const uint32_t ve = (q == n) ? vf : kOOB;
-          bstore(rf[j], ve, fo1, fmaf(a.fscale, gq1, bload(rf[j], vv, fo1)));
+          bstore(rf[j], ve, fo1, fmaf(a.fscale, gq1, add ? bload(rf[j], vv, fo1) : 0.0f));
         }
       }
     }
@@ -488,8 +489,11 @@ __device__ __forceinline__ int xcd_block(const int b, const int nblocks) {
   return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+#ifndef F3D_STREAM_WAVES_PER_SIMD
+#define F3D_STREAM_WAVES_PER_SIMD 1
+#endif
 template <int DIR, int LIM, bool COMP, int NST, int BPT, bool FOUT = false>
-__global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
+__global__ void __launch_bounds__(512, F3D_STREAM_WAVES_PER_SIMD) k_stream(SweepArgs a) {
   stream_body<DIR, LIM, COMP, NST, BPT, FOUT>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
@@ -507,6 +511,18 @@ __global__ void __launch_bounds__(512) k_stream_batch(SweepBatch b) {
 // =============================================================================
 // k_sweep_x: x sweep, one wave per line, C cells per lane
 // =============================================================================
+
+#ifndef F3D_X_DPP
+#define F3D_X_DPP 0
+#endif
+// whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): the value of
+// lane l-1 / l+1; the lane shifted in (0 / 63) reads 0
+__device__ __forceinline__ float wave_shr1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_shl1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
 
 template <int C, int LIM, bool COMP, int NST, bool FOUT = false>
 __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, const int by) {
@@ -643,11 +659,20 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
 
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
-      // neighbours' edge cells (same wave: cross-lane permute, no barrier)
+      // neighbours' edge cells (same wave, no barrier): lane l reads lane
+      // l-1 (left) and l+1 (right); lane 0 / 63 are always first / last
+      // segments, whose values are replaced by the ghosts below
+#if F3D_X_DPP
+      float yl2 = wave_shr1(y[C - 2]);
+      float yl1 = wave_shr1(y[C - 1]);
+      float yr0 = wave_shl1(y[0]);
+      float yr1 = wave_shl1(y[1]);
+#else
       float yl2 = __shfl(y[C - 2], lsrc, 64);
       float yl1 = __shfl(y[C - 1], lsrc, 64);
       float yr0 = __shfl(y[0], rsrc, 64);
       float yr1 = __shfl(y[1], rsrc, 64);
+#endif
       if (first) {
         yl2 = gL;
         yl1 = gL;
@@ -702,7 +727,8 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
 #pragma unroll
       for (int q = 0; q <= C; ++q) {
         const int p = s0 + q;
-        if (act && ((q < C && p <= n + 1) || (q == C && p == n + 1))) fo[p] = fmaf(a.fscale, gh[q], fo[p]);
+        if (act && ((q < C && p <= n + 1) || (q == C && p == n + 1)))
+          fo[p] = fmaf(a.fscale, gh[q], a.fout_add ? fo[p] : 0.0f);
       }
     }
     float tl0 = 0.0f, trn = 0.0f;

@@ -75,13 +75,14 @@ def rel_l1(a, ref):
 
 
 def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", gain=1.0, fine_fp32=False,
-             gains=None, **kw):
+             gains=None, variants=None, **kw):
     """Run fine (2 nsteps steps), coarse plain and coarse learned (nsteps steps
     each) from the same initial state; compare the coarse column loads and
     ground deposits with the fine ones averaged onto the coarse grid.
     fine_fp32=True also runs the fine grid with the FP32 state and compares it
     the same way (when the truth uses the compensated state). gains: several
-    correction gains at once (keys "learned_<gain>").
+    correction gains at once (keys "learned_<gain>"); variants: name ->
+    keyword arguments of ADS.set_learned (smin, gain, flux_limit).
     Returns a dict of relative L1 errors."""
     nx, ny, nz, nb = size
     fine, coarse = make_pair(nx, ny, nz, nb, seed, state=truth_state, **kw)
@@ -104,18 +105,20 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", g
         out["fine_fp32"] = {"load_l1": rel_l1(_pairs_mean(column_load(c2, nx, ny, nz, dz)), lf),
                             "deposit_l1": rel_l1(_coarse_mean(a2, nx, ny).sum(axis=-1), dep_f)}
         del c2, a2
-    runs = [("plain", None, 1.0)]
-    if gains is None:
-        runs.append(("learned", net, gain))
+    runs = [("plain", None, {})]
+    if variants is not None:
+        runs += [(k, net, v) for k, v in variants.items()]
+    elif gains is None:
+        runs.append(("learned", net, {"smin": smin, "gain": gain}))
     else:
-        runs += [(f"learned_{g}", net, g) for g in gains]
-    for name, learned, gn in runs:
+        runs += [(f"learned_{g}", net, {"smin": smin, "gain": g}) for g in gains]
+    for name, learned, opts in runs:
         pc = config1_wind_problem(nx, ny, nz, nb, kw.get("dx", 5000.0), dz, seed, coarsen=2,
                                   **{k: v for k, v in kw.items() if k not in ("dx", "dz")})
         g = ads.ADS(pc.nx, pc.ny, pc.nz, pc.nb, "minmod", "rk", 8, "fp32")
         g.set_problem(pc)
         del pc
-        g.set_learned(learned, smin, gn)
+        g.set_learned(learned, **opts) if learned is not None else g.set_learned(None)
         g.run(dt, nsteps, 0)
         cc, ac = g.get_state()
         g.release()

@@ -283,10 +283,12 @@ int fall3d_mlp_train(fall3d_mlp *net, int steps, int batch, float lr, unsigned l
  * axis 0 (u layout x nbins) or 1 (v layout x nbins) to the host. */
 int fall3d_ads_set_flux_output(fall3d_ads_ctx *ctx, int on);
 int fall3d_ads_get_face_flux(fall3d_ads_ctx *ctx, int axis, float *out);
-/* Use net (borrowed; NULL: plain KT) in every later step: faces whose
- * stencil maximum is <= smin get no correction; gain scales the applied
- * correction (1: as predicted). */
-int fall3d_ads_set_learned(fall3d_ads_ctx *ctx, fall3d_mlp *net, double smin, double gain);
+/* Use net (borrowed; NULL: plain KT) in every later step. Faces whose
+ * stencil maximum is <= smin get no correction (smin < 0: <= |smin| times the
+ * field's largest value, found on the device each step); gain scales the
+ * applied correction (1: as predicted); flux_limit > 0 bounds each face's
+ * correction by flux_limit times the plain step's flux through that face. */
+int fall3d_ads_set_learned(fall3d_ads_ctx *ctx, fall3d_mlp *net, double smin, double gain, double flux_limit);
 /* Interior of coarse := 2x2 horizontal means of fine (fine = 2x coarse). */
 int fall3d_ads_coarse_grain(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse);
 /* One training sample of the whole coarse grid: coarse := coarse-grained
