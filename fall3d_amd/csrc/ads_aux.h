@@ -6,20 +6,23 @@
 namespace f3d {
 
 // Up to 16 reductions of sweep partials (4 doubles per block) into acc[].
+// Up to 8 reductions of per-workgroup partials (4 doubles per workgroup)
+// into acc[]: job j adds sum_b part[b*4 + s] to acc[dst[j][s]] for
+// s < nslot[j].
 struct ReduceJob {
-  const double *part[16];
-  int nblocks[16];
-  int slot[16];
-  int dst[16];
+  const double *part[8];
+  int nblocks[8];
+  int nslot[8];
+  int dst[8][4];
   int count;
 };
 
-// One reduction of an ensemble step: dst += sum_b part[b*4 + slot].
+// One reduction of an ensemble step: dst[s] += sum_b part[b*4 + s], s < nslot.
 struct BatchJob {
   const double *part;
-  double *dst;
+  double *dst[4];
   int nblocks;
-  int slot;
+  int nslot;
 };
 
 // Sparse source of one ensemble member (see source_inject); the injected

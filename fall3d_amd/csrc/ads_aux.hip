@@ -293,55 +293,78 @@ hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double
 
 // One wave per (array, slot): sums partials[b*4+slot] over b in a fixed
 // lane-strided order, then a fixed shuffle tree; adds into acc[dst].
-// Sum of p[b*4 + slot] over b < nb by one 256-thread workgroup, in a fixed
-// order: thread t keeps 8 running sums over b = t + 256 (8 k + r) (8
-// independent loads in flight), combines them r = 0..7, then a shuffle tree
-// per wave and the 4 wave sums in order. Valid in thread 0.
-__device__ __forceinline__ double block_slot_sum(const double *__restrict__ p, int nb, int slot) {
-  __shared__ double red[4];
+// Sums of the 4 slots of p[b*4 + s] over b < nb by one 1024-thread
+// workgroup, in a fixed order: thread t reads the 32-byte rows b = t + 1024 k
+// (4 rows in flight, kept in separate sums and combined in order), then a
+// shuffle tree per wave and the 16 wave sums in order. Valid in thread 0.
+__device__ __forceinline__ void block_rows_sum(const double *__restrict__ p, int nb, double out[4]) {
+  __shared__ double red[16][4];
   const int t = threadIdx.x;
-  double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int b0 = 0; b0 < nb; b0 += 2048) {
+  double s[4][4];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int b = b0 + 256 * r + t;
-      if (b < nb) s[r] += p[(size_t)b * 4 + slot];
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[u][k] = 0.0;
+  for (int b0 = 0; b0 < nb; b0 += 4096) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + 1024 * u + t;
+      if (b < nb) {
+        const double2 lo = *reinterpret_cast<const double2 *>(p + (size_t)b * 4);
+        const double2 hi = *reinterpret_cast<const double2 *>(p + (size_t)b * 4 + 2);
+        s[u][0] += lo.x;
+        s[u][1] += lo.y;
+        s[u][2] += hi.x;
+        s[u][3] += hi.y;
+      }
     }
   }
-  double v = s[0];
 #pragma unroll
-  for (int r = 1; r < 8; ++r) v += s[r];
-  v = wsum(v);
-  if ((t & 63) == 0) red[t >> 6] = v;
+  for (int k = 0; k < 4; ++k) {
+    double v = ((s[0][k] + s[1][k]) + s[2][k]) + s[3][k];
+    v = wsum(v);
+    if ((t & 63) == 0) red[t >> 6][k] = v;
+  }
   __syncthreads();
-  return ((red[0] + red[1]) + red[2]) + red[3];
+  if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double v = 0.0;
+      for (int w = 0; w < 16; ++w) v += red[w][k];
+      out[k] = v;
+    }
+  }
 }
 
-// one workgroup (256 threads) per job
-__global__ void __launch_bounds__(256) k_finalize(double *__restrict__ acc, ReduceJob jobs) {
+// one workgroup (1024 threads) per job
+__global__ void __launch_bounds__(1024) k_finalize(double *__restrict__ acc, ReduceJob jobs) {
   const int j = blockIdx.x;
-  const double s = block_slot_sum(jobs.part[j], jobs.nblocks[j], jobs.slot[j]);
-  if (threadIdx.x == 0) acc[jobs.dst[j]] += s;
+  double s[4];
+  block_rows_sum(jobs.part[j], jobs.nblocks[j], s);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < jobs.nslot[j]; ++k) acc[jobs.dst[j][k]] += s[k];
 }
 
 // Ensemble form: one workgroup per job of a device job table.
-__global__ void __launch_bounds__(256) k_finalize_batch(const BatchJob *__restrict__ jobs, int njobs) {
+__global__ void __launch_bounds__(1024) k_finalize_batch(const BatchJob *__restrict__ jobs) {
   const BatchJob job = jobs[blockIdx.x];
-  const double s = block_slot_sum(job.part, job.nblocks, job.slot);
-  if (threadIdx.x == 0) *job.dst += s;
+  double s[4];
+  block_rows_sum(job.part, job.nblocks, s);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < job.nslot; ++k) *job.dst[k] += s[k];
 }
 
 hipError_t finalize_batch(const BatchJob *jobs, int njobs, hipStream_t st) {
   (void)hipGetLastError();
   if (njobs == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize_batch, dim3(njobs), dim3(256), 0, st, jobs, njobs);
+  hipLaunchKernelGGL(k_finalize_batch, dim3(njobs), dim3(1024), 0, st, jobs);
   return hipGetLastError();
 }
 
 hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
   if (jobs.count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3(jobs.count), dim3(256), 0, st, acc, jobs);
+  hipLaunchKernelGGL(k_finalize, dim3(jobs.count), dim3(1024), 0, st, acc, jobs);
   return hipGetLastError();
 }
 

@@ -87,7 +87,7 @@ void drain_events(fall3d_ads_ctx *c) {
 }
 
 // ---- tiling -------------------------------------------------------------------
-const int kCpt[] = {2, 3, 4, 6, 8, 12, 16};
+const int kCpt[] = {2, 3, 4, 6, 8, 12, 16, 20};
 
 Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
   Tiling t;
@@ -100,11 +100,11 @@ Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
       C = c->user_cpt[0];
     } else {
       for (int cc : kCpt) {
-        if ((n + cc - 1) / cc <= 64) { C = cc; break; }
+        if (n / cc + 1 <= 64) { C = cc; break; }
       }
     }
     t.cpt = C;
-    t.S = (n + C - 1) / C;
+    t.S = n / C + 1;  // the last segment holds face n+1 (possibly no cell)
     if (t.S > 64) t.S = 0;  // unsupported; caught by the caller
     t.block = 256;
     const int lpw = t.S ? 64 / t.S : 1;
@@ -145,7 +145,7 @@ int refresh_tiling(fall3d_ads_ctx *c) {
   for (int d = 0; d < 3; ++d) {
     Tiling t = choose_tiling(c, d);
     if (d == 0 && (t.S <= 0 || t.cpt < 2 || !sweep_cpt_supported(t.cpt)))
-      return fail(c, FALL3D_ADS_EINVAL, "no supported x tiling (nx=%d cpt=%d; nx <= 1024 needs cpt <= 16)",
+      return fail(c, FALL3D_ADS_EINVAL, "no supported x tiling (nx=%d cpt=%d; nx < 64 cpt is required)",
                   axis_n(c, d), t.cpt);
     c->tl[d] = t;
     const size_t need = (d == 0) ? (size_t)t.gx * t.groups : (size_t)t.gx;
@@ -263,13 +263,11 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
 void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks) {
   // partial slots: 0 ref beg, 1 ref end, 2 ledger beg, 3 ledger end
   const int dst[4] = {ACC_FLUX + 2 * dir, ACC_FLUX + 2 * dir + 1, ACC_LEDGER + 2 * dir, ACC_LEDGER + 2 * dir + 1};
-  for (int s = 0; s < 4; ++s) {
-    j.part[j.count] = c->part[dir + 1];
-    j.nblocks[j.count] = nblocks;
-    j.slot[j.count] = s;
-    j.dst[j.count] = dst[s];
-    j.count++;
-  }
+  j.part[j.count] = c->part[dir + 1];
+  j.nblocks[j.count] = nblocks;
+  j.nslot[j.count] = 4;
+  for (int s = 0; s < 4; ++s) j.dst[j.count][s] = dst[s];
+  j.count++;
 }
 
 int run_finalize(fall3d_ads_ctx *c, const ReduceJob &j) {
@@ -858,8 +856,8 @@ int fall3d_ads_step(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
     }
     j.part[j.count] = c->src_part;
     j.nblocks[j.count] = nblk;
-    j.slot[j.count] = 0;
-    j.dst[j.count] = ACC_INJECTED;
+    j.nslot[j.count] = 1;
+    j.dst[j.count][0] = ACC_INJECTED;
     j.count++;
   }
   SweepOut so;
@@ -1030,7 +1028,7 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
     std::vector<int> cand;
     if (d == 0) {
       for (int i = 0; i < kSweepCptCount; ++i)
-        if ((c->nx + kSweepCpt[i] - 1) / kSweepCpt[i] <= 64) cand.push_back(kSweepCpt[i]);
+        if (c->nx / kSweepCpt[i] + 1 <= 64) cand.push_back(kSweepCpt[i]);
     } else {
       for (int g = 1; g <= std::min(c->nb, 8); ++g) {
         const int groups = (c->nb + g - 1) / g;
@@ -1046,10 +1044,19 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
       const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
       const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
       const float *kd = d == 0 ? c->k1 : (d == 1 ? c->k2 : c->k3);
-      SweepOut so;
-      if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so))) break;
+      // a candidate costs its sweep plus the reduction of its partials
+      auto sweep_and_reduce = [&]() -> int {
+        SweepOut so;
+        int r = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so);
+        if (r) return r;
+        ReduceJob j;
+        memset(&j, 0, sizeof(j));
+        add_sweep_jobs(j, c, d, so.nblocks);
+        return run_finalize(c, j);
+      };
+      if ((rc = sweep_and_reduce())) break;
       (void)hipEventRecord(ea, c->stream);
-      for (int r = 0; r < reps && !rc; ++r) rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so);
+      for (int r = 0; r < reps && !rc; ++r) rc = sweep_and_reduce();
       (void)hipEventRecord(eb, c->stream);
       if (rc) break;
       (void)hipEventSynchronize(eb);

@@ -18,8 +18,8 @@ hipError_t launch_sweep_z_batch(int bpt, int limiter, bool comp, int nst, const 
                                 dim3 block, hipStream_t st);
 
 // cells per lane with an instantiated x-sweep kernel (see F3D_CPT_LIST)
-const int kSweepCpt[] = {2, 3, 4, 6, 8, 12, 16};
-const int kSweepCptCount = 7;
+const int kSweepCpt[] = {2, 3, 4, 6, 8, 12, 16, 20};
+const int kSweepCptCount = 8;
 
 bool sweep_cpt_supported(int cpt) {
   for (int i = 0; i < kSweepCptCount; ++i)

@@ -181,12 +181,26 @@ int build_tables(fall3d_ads_ensemble *e, const double *dt, const double *stime) 
     std::vector<BatchJob> jobs;
     for (int m = 0; m < M; ++m) {
       fall3d_ads_ctx *c = e->mem[m];
-      if (c->nsrc > 0) jobs.push_back({c->src_part, c->acc + ACC_INJECTED, (c->nsrc + 255) / 256, 0});
+      if (c->nsrc > 0) {
+        BatchJob jb;
+        memset(&jb, 0, sizeof(jb));
+        jb.part = c->src_part;
+        jb.dst[0] = c->acc + ACC_INJECTED;
+        jb.nblocks = (c->nsrc + 255) / 256;
+        jb.nslot = 1;
+        jobs.push_back(jb);
+      }
       for (int d = 0; d < 3; ++d) {
         const Tiling &t = c->tl[d];
-        const int nbl = (d == 0) ? t.gx * t.groups : t.gx;
-        const int dst[4] = {ACC_FLUX + 2 * d, ACC_FLUX + 2 * d + 1, ACC_LEDGER + 2 * d, ACC_LEDGER + 2 * d + 1};
-        for (int s = 0; s < 4; ++s) jobs.push_back({c->part[d + 1], c->acc + dst[s], nbl, s});
+        BatchJob jb;
+        jb.part = c->part[d + 1];
+        jb.dst[0] = c->acc + ACC_FLUX + 2 * d;
+        jb.dst[1] = c->acc + ACC_FLUX + 2 * d + 1;
+        jb.dst[2] = c->acc + ACC_LEDGER + 2 * d;
+        jb.dst[3] = c->acc + ACC_LEDGER + 2 * d + 1;
+        jb.nblocks = (d == 0) ? t.gx * t.groups : t.gx;
+        jb.nslot = 4;
+        jobs.push_back(jb);
       }
     }
     e->njobs = (int)jobs.size();
@@ -239,7 +253,7 @@ void default_tiling(fall3d_ads_ensemble *e) {
   for (auto *c : e->mem) nxmax = std::max(nxmax, c->nx);
   e->x_cpt = 0;
   for (int i = 0; i < kSweepCptCount; ++i)
-    if ((nxmax + kSweepCpt[i] - 1) / kSweepCpt[i] <= 64) {
+    if (nxmax / kSweepCpt[i] + 1 <= 64) {
       e->x_cpt = kSweepCpt[i];
       break;
     }
@@ -385,7 +399,7 @@ int fall3d_ads_ensemble_set_tiling(fall3d_ads_ensemble *e, int x_cells_per_lane,
     if (!sweep_cpt_supported(x_cells_per_lane))
       return efail(e, FALL3D_ADS_EINVAL, "unsupported cells per lane %d", x_cells_per_lane);
     for (auto *c : e->mem)
-      if ((c->nx + x_cells_per_lane - 1) / x_cells_per_lane > 64)
+      if (c->nx / x_cells_per_lane + 1 > 64)
         return efail(e, FALL3D_ADS_EINVAL, "cells per lane %d too small for nx = %d", x_cells_per_lane, c->nx);
     e->x_cpt = x_cells_per_lane;
   }
@@ -498,7 +512,7 @@ int fall3d_ads_ensemble_autotune(fall3d_ads_ensemble *e, const double *dt, const
   for (auto *c : e->mem) nxmax = std::max(nxmax, c->nx);
   std::vector<int> xc, yz;
   for (int i = 0; i < kSweepCptCount; ++i)
-    if ((nxmax + kSweepCpt[i] - 1) / kSweepCpt[i] <= 64) xc.push_back(kSweepCpt[i]);
+    if (nxmax / kSweepCpt[i] + 1 <= 64) xc.push_back(kSweepCpt[i]);
   for (int g = 1; g <= std::min(e->nb, 8); ++g) {
     const int groups = (e->nb + g - 1) / g;
     if ((e->nb + groups - 1) / groups == g) yz.push_back(g);  // distinct shapes only

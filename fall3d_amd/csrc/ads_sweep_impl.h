@@ -489,11 +489,8 @@ __device__ __forceinline__ int xcd_block(const int b, const int nblocks) {
   return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
-#ifndef F3D_STREAM_WAVES_PER_SIMD
-#define F3D_STREAM_WAVES_PER_SIMD 1
-#endif
 template <int DIR, int LIM, bool COMP, int NST, int BPT, bool FOUT = false>
-__global__ void __launch_bounds__(512, F3D_STREAM_WAVES_PER_SIMD) k_stream(SweepArgs a) {
+__global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
   stream_body<DIR, LIM, COMP, NST, BPT, FOUT>(a, xcd_block(blockIdx.x, gridDim.x));
 }
 
@@ -512,21 +509,11 @@ __global__ void __launch_bounds__(512) k_stream_batch(SweepBatch b) {
 // k_sweep_x: x sweep, one wave per line, C cells per lane
 // =============================================================================
 
-#ifndef F3D_X_DPP
-#define F3D_X_DPP 0
-#endif
-// whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): the value of
-// lane l-1 / l+1; the lane shifted in (0 / 63) reads 0
-__device__ __forceinline__ float wave_shr1(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float wave_shl1(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
-}
-
 template <int C, int LIM, bool COMP, int NST, bool FOUT = false>
 __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, const int by) {
   static_assert(C >= 2, "segments need two edge cells");
+  // Segmentation: S = n/C + 1 segments of C cells, so that face n+1 always
+  // lies inside a segment (faces 0..C-1 of each lane are computed locally).
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int S = a.S, n = a.n;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -551,12 +538,12 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   const float dt = a.dt;
 
   // wind and diffusion of this line, read once for all bins
-  float up[C + 1], um[C + 1], kf[C + 1], hb[C], db[C], fb[C];
+  float up[C], um[C], kf[C], hb[C], db[C], fb[C];
   float vbeg = 0.0f, vend = 0.0f;
   {
     float kprev = act ? a.kd[kbase + (size_t)s0] : 0.0f;  // cell s0-1 (stored at index s0)
 #pragma unroll
-    for (int q = 0; q <= C; ++q) {
+    for (int q = 0; q < C; ++q) {
       int f = s0 + q;
       f = f > n + 2 ? n + 2 : f;
       float v = 0.0f, kc = 0.0f, h = 0.0f;
@@ -570,13 +557,16 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       kf[q] = (kprev + kc) * h;
       kprev = kc;
     }
+    // stage multipliers are zero beyond cell n: the ghost positions of the
+    // last segment then keep their frozen value through every stage
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       int p = s0 + q;
+      const bool in = act && p <= n;
       p = p > n + 2 ? n + 2 : p;
-      db[q] = act ? a.dtib[p] : 0.0f;
-      hb[q] = act ? a.hdtib[p] : 0.0f;
-      fb[q] = act ? a.ftib[p] : 0.0f;
+      db[q] = in ? a.dtib[p] : 0.0f;
+      hb[q] = in ? a.hdtib[p] : 0.0f;
+      fb[q] = in ? a.ftib[p] : 0.0f;
     }
     if (act) {
       vbeg = s1 * a.vel1[vbase + 1] + s * a.vel2[vbase + 1];
@@ -620,11 +610,14 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
         }
       }
     }
-    // free-flow ghosts (Sources/mod_ADS.F90:1239-1265), frozen for the sweep
+    // free-flow ghosts (Sources/mod_ADS.F90:1239-1265), frozen for the sweep.
+    // Cell n is in the last segment, or (when that segment starts at n+1)
+    // the last cell of the segment to its left.
     float gL = 0.0f, gR = 0.0f;
+    const float cleft = __shfl(cn[C - 1], lsrc, 64);
     if (first) gL = (vbeg > 0.0f) ? (2.0f * 0.0f - cn[0]) : cn[0];
     if (last) {
-      float cl = cn[0];
+      float cl = (qn < 0) ? cleft : cn[0];
 #pragma unroll
       for (int q = 1; q < C; ++q) cl = (q == qn) ? cn[q] : cl;
       gR = (vend < 0.0f) ? (2.0f * 0.0f - cl) : cl;
@@ -659,41 +652,27 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
 
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
-      // neighbours' edge cells (same wave, no barrier): lane l reads lane
-      // l-1 (left) and l+1 (right); lane 0 / 63 are always first / last
-      // segments, whose values are replaced by the ghosts below
-#if F3D_X_DPP
-      float yl2 = wave_shr1(y[C - 2]);
-      float yl1 = wave_shr1(y[C - 1]);
-      float yr0 = wave_shl1(y[0]);
-      float yr1 = wave_shl1(y[1]);
-#else
+      // neighbours' edge cells (same wave: cross-lane permute, no barrier)
       float yl2 = __shfl(y[C - 2], lsrc, 64);
       float yl1 = __shfl(y[C - 1], lsrc, 64);
       float yr0 = __shfl(y[0], rsrc, 64);
-      float yr1 = __shfl(y[1], rsrc, 64);
-#endif
       if (first) {
         yl2 = gL;
         yl1 = gL;
       }
-      if (last) {
-        yr0 = gR;
-        yr1 = gR;
-      }
-      float w[C + 4];
+      if (last) yr0 = gR;
+      float w[C + 3];
       w[0] = yl2;
       w[1] = yl1;
 #pragma unroll
       for (int q = 0; q < C; ++q) w[q + 2] = y[q];
       w[C + 2] = yr0;
-      w[C + 3] = yr1;
       const bool last_stage = (st == NST - 1);
       float d_cur = w[2] - w[1];
       float m_prev = limiter_slope<LIM>(w[1] - w[0], d_cur);
-      float G_prev = 0.0f;
+      float G_prev = 0.0f, G0 = 0.0f;
 #pragma unroll
-      for (int q = 0; q <= C; ++q) {
+      for (int q = 0; q < C; ++q) {
         const float d_next = w[q + 3] - w[q + 2];
         const float m_cur = limiter_slope<LIM>(d_cur, d_next);
         const float cL = fmaf(0.5f, m_prev, w[q + 1]);
@@ -701,6 +680,7 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
         // G = F - P with F = max(u,0) cL + min(u,0) cR
         const float P = kf[q] * d_cur;
         const float G = fmaf(up[q], cL, fmaf(um[q], cR, -P));
+        if (q == 0) G0 = G;
         if (last_stage) {  // reference boundary fluxes F + P
           if (q == 0) RFb = fmaf(2.0f, P, G);
           if (q == qn + 1) RFe = fmaf(2.0f, P, G);
@@ -713,22 +693,26 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
         else
           gh[q] = fmaf(2.0f, G, gh[q]);
         if (q > 0 && !last_stage) {
-          const float yn = fmaf((st == 2) ? db[q - 1] : hb[q - 1], G_prev - G, cn[q - 1]);
-          y[q - 1] = (last && q - 1 > qn) ? gR : yn;
+          y[q - 1] = fmaf((st == 2) ? db[q - 1] : hb[q - 1], G_prev - G, cn[q - 1]);
         }
         G_prev = G;
         d_cur = d_next;
         m_prev = m_cur;
       }
+      // Face C is the right neighbour's face 0 (same inputs, same arithmetic:
+      // bit-identical). The last segment's face C lies beyond n+1 and only
+      // feeds frozen ghost positions (zero multipliers).
+      const float GC = __shfl(G0, rsrc, 64);
+      if (!last_stage) y[C - 1] = fmaf((st == 2) ? db[C - 1] : hb[C - 1], G_prev - GC, cn[C - 1]);
     }
+    gh[C] = __shfl(gh[0], rsrc, 64);
 
     if constexpr (FOUT) {  // face mass flux per unit area: faces s0 .. s0+C-1, and n+1
       float *fo = a.fout + (size_t)(nx + 3) * ny * nz * b + vbase;
 #pragma unroll
-      for (int q = 0; q <= C; ++q) {
+      for (int q = 0; q < C; ++q) {
         const int p = s0 + q;
-        if (act && ((q < C && p <= n + 1) || (q == C && p == n + 1)))
-          fo[p] = fmaf(a.fscale, gh[q], a.fout_add ? fo[p] : 0.0f);
+        if (act && p <= n + 1) fo[p] = fmaf(a.fscale, gh[q], a.fout_add ? fo[p] : 0.0f);
       }
     }
     float tl0 = 0.0f, trn = 0.0f;
@@ -756,8 +740,8 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
         acc_rb += (double)RFb * area * (double)dt;
         acc_lb -= (double)tl0 * a.dxb64[1] * area;
       }
-      if (seg == S - 1) {
-        acc_re += (double)RFe * area * (double)dt;
+      if (qn >= -1 && qn <= C - 2) acc_re += (double)RFe * area * (double)dt;  // holds face n+1
+      if (qn >= 0 && qn <= C - 1) {  // holds cell n
         acc_le += (double)trn * a.dxb64[n] * area;
       }
     }

@@ -4,7 +4,7 @@
 
 namespace f3d {
 
-#define F3D_CPT_LIST(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16)
+#define F3D_CPT_LIST(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(20)
 
 template <int C>
 static hipError_t launch_x_c(int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid, dim3 block,

@@ -254,3 +254,26 @@ def test_operational_full_size_two_steps():
             err = abs(b["mass_cells"] + b["ledger"].sum() - b["injected"]) / b["injected"]
             assert err <= 1e-12
         g.release()
+
+
+@pytest.mark.parametrize("nx,cpt", [(12, 4), (12, 3), (13, 4), (16, 16), (64, 2), (63, 2), (40, 20)])
+def test_x_segmentation_edge_cases(nx, cpt):
+    """x sweep with every segmentation case: cell n at the end of a segment
+    (the last segment then holds only ghosts and face n+1), in its middle,
+    one segment per line, and 64-lane lines; against the oracle."""
+    print(f"case nx={nx} cpt={cpt}", flush=True)
+    p = random_problem(nx, 7, 5, 2, seed=nx + cpt)
+    o = Oracle(p, 1, 2)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, 1, 2, 8, "fp32", grid=p.grid)
+    g.set_tiling(0, cells_per_thread=cpt)
+    u, v, w = g.get_vcomps()
+    u[...] = p.u1
+    v[...] = p.v1
+    dt = _dt(p, o)
+    cr, cg = p.new_c(), p.new_c()
+    o.solve_x(dt, cr, p.k1, p.u1)
+    W, E = g.solve_along_x(0.0, 0.0, dt, cg, p.k1)
+    check_field(cg, cr, "x sweep c")
+    scale = _flux_scale(p, dt)
+    assert abs(W - o.flux[0]) <= 1e-5 * scale and abs(E - o.flux[1]) <= 1e-5 * scale
+    g.release()
