@@ -138,6 +138,29 @@ __global__ void k_dt_over_dxb(const double *__restrict__ dxb, int n3, double dt,
   }
 }
 
+__global__ void k_iter_table(const float *__restrict__ tab, const float *__restrict__ hdxp, int n, int nst,
+                             float *__restrict__ itab) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= iter_rows(n)) return;
+  const int n3 = n + 3;
+  auto cl = [&](int p) { return p < 0 ? 0 : (p > n + 2 ? n + 2 : p); };
+  const float *dtib = tab, *hdtib = tab + n3, *ftib = tab + 2 * n3;
+  float *r = itab + (size_t)t * 8;
+  r[0] = hdxp[cl(t - 2)];
+  r[1] = hdtib[cl(t - 3)];
+  r[2] = hdtib[cl(t - 5)];
+  r[3] = dtib[cl(t - 7)];
+  r[4] = ftib[cl(t - 2 * nst - 1)];
+  r[5] = r[6] = r[7] = 0.0f;
+}
+
+hipError_t iter_table(const float *tab, const float *hdxp, int n, int nst, float *itab, hipStream_t st) {
+  (void)hipGetLastError();
+  const int rows = iter_rows(n);
+  hipLaunchKernelGGL(k_iter_table, dim3((rows + 255) / 256), dim3(256), 0, st, tab, hdxp, n, nst, itab);
+  return hipGetLastError();
+}
+
 hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, hipStream_t st) {
   (void)hipGetLastError();
   const double wfinal = (tm == 1) ? 1.0 : 1.0 / 6.0;

@@ -169,6 +169,7 @@ int ensure_dt(fall3d_ads_ctx *c, double dt) {
   if (c->dtib_ok && c->dtib_dt == dt) return FALL3D_ADS_OK;
   for (int d = 0; d < 3; ++d) {
     hipError_t e = dt_over_dxb(c->dxb64[d], axis_n(c, d) + 3, dt, c->tm, c->dttab[d], c->stream);
+    if (e == hipSuccess) e = iter_table(c->dttab[d], c->hdxp[d], axis_n(c, d), c->tm == 1 ? 1 : 4, c->itab[d], c->stream);
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "dt/dxb: %s", hipGetErrorString(e));
   }
   c->dtib_dt = dt;
@@ -209,6 +210,7 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
     a.dtib = c->dttab[dir];
     a.hdtib = c->dttab[dir] + n3;
     a.ftib = c->dttab[dir] + 2 * n3;
+    a.itab = c->itab[dir];
   }
   a.dxb64 = c->dxb64[dir];
   if (dir == 0) {
@@ -523,6 +525,7 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     HIPCHK(c, dalloc(&c->hdxp[d], n + 3));
     HIPCHK(c, dalloc(&c->idxb[d], n + 3));
     HIPCHK(c, dalloc(&c->dttab[d], 3 * (n + 3)));
+    HIPCHK(c, dalloc(&c->itab[d], (size_t)8 * iter_rows(n)));
   }
   const size_t ksz = (size_t)real_kind;
   c->hu = calloc(n_u(c), ksz);
@@ -555,6 +558,7 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
     dfree(c->hdxp[d]);
     dfree(c->idxb[d]);
     dfree(c->dttab[d]);
+    dfree(c->itab[d]);
   }
   free(c->hu);
   free(c->hv);

@@ -64,6 +64,7 @@ struct fall3d_ads_ctx {
   float *hdxp[3] = {nullptr, nullptr, nullptr};
   float *idxb[3] = {nullptr, nullptr, nullptr};
   float *dttab[3] = {nullptr, nullptr, nullptr};  // dt/dxb, 0.5 dt/dxb, final factor (for dtib_dt)
+  float *itab[3] = {nullptr, nullptr, nullptr};   // per-iteration factors of the streaming sweeps
   double dtib_dt = 0.0;
   bool dtib_ok = false;
   std::vector<double> hdxp64[3], hdxb64[3];

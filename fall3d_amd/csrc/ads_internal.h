@@ -42,6 +42,7 @@ struct SweepArgs {
   const float *dtib;         // dt/dxb(p), p = 0..n+2 (for the launch's dt)
   const float *hdtib;        // 0.5*dt/dxb(p)
   const float *ftib;         // final-update factor: dt/(6 dxb(p)) for RK4, dt/dxb(p) for Euler
+  const float *itab;         // streaming sweeps: 8 floats per pipeline iteration (see iter_table)
   const double *dxb64;       // dxb(p) in FP64, p = 0..n+2 (mass measure of the ledger)
   const double *areaA;       // FP64 transverse cell sizes for the face area (index 0..)
   const double *areaB;
@@ -63,6 +64,13 @@ struct SweepBatch {
 // Table of one axis for time step dt (ads_aux.hip), 3 (n+3) floats:
 // [dt/dxb | 0.5 dt/dxb | w dt/dxb], w = 1/6 (RK4) or 1 (Euler, tm == 1).
 hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, hipStream_t st);
+// Per-iteration factors of the streaming sweep of an axis with n cells and
+// nst stages, rows t = 0 .. iter_rows(n) - 1 of 8 floats:
+//   0.5/dxp of face t-2, 0.5 dt/dxb of cells t-3 and t-5, dt/dxb of cell
+//   t-7, final factor of cell t-2 nst-1 (indices clamped to 0..n+2).
+// tab: the dt_over_dxb table of the axis; hdxp: 0.5/dxp(f-1) at face f.
+__host__ __device__ inline int iter_rows(int n) { return n + 2 * 4 + 4; }
+hipError_t iter_table(const float *tab, const float *hdxp, int n, int nst, float *itab, hipStream_t st);
 
 // Sweep launchers (ads_sweep_*.hip). cpt = cells per lane (x sweep) or
 // bins per thread (y and z sweeps, 1 or 2).

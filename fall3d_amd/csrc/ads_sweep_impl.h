@@ -205,10 +205,12 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   const rsrc_t rv1 = make_rsrc(a.vel1, nvel * 4);
   const rsrc_t rv2 = make_rsrc(a.vel2, nvel * 4);
   const rsrc_t rk = make_rsrc(a.kd, nkd * 4);
-  cfloat *hdxp = (cfloat *)a.hdxp;
-  cfloat *dtib = (cfloat *)a.dtib;
-  cfloat *hdtib = (cfloat *)a.hdtib;
-  cfloat *ftib = (cfloat *)a.ftib;
+  // per-iteration factors, one 8-float row per iteration (scalar loads,
+  // fetched one iteration ahead)
+  typedef float f8 __attribute__((ext_vector_type(8)));
+  typedef __attribute__((address_space(4))) const f8 cf8;
+  cf8 *itab = (cf8 *)a.itab;
+  f8 rnext = itab[0];
 
   // lane byte offsets of position -1 (c), face 0 (winds), cell -1 (K); byte strides
   uint32_t vc, cs4, vv, vst4, vk, kst4;
@@ -327,6 +329,8 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   // selects are compiled out.
   auto step = [&](auto gen_tag, const int t, const int u) {
     constexpr bool GEN = decltype(gen_tag)::value;
+    const f8 row = rnext;  // [0.5/dxp(t-2), 0.5dt/dxb(t-3), 0.5dt/dxb(t-5), dt/dxb(t-7), final(t-2NST-1)]
+    rnext = itab[t + 1];
     // ---- prefetch cell t+kPF-1 and face t+kPF-2
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
@@ -355,7 +359,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
         up[j][sl] = fmaxf(ve, 0.0f);
         um[j][sl] = fminf(ve, 0.0f);
       }
-      kf[sl] = (rkk[(u - 3) & 7] + rkk[sl]) * hdxp[clampi(t - 2, 0, n + 2)];
+      kf[sl] = (rkk[(u - 3) & 7] + rkk[sl]) * row[0];
     }
     // ---- the pipeline: stage st consumes position t+1-2(st+1)
 #pragma unroll
@@ -400,7 +404,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
         sm[j][st] = m;
         sG[j][st] = G;
         if (st < NST - 1) {
-          const float hib = (st == 2) ? dtib[clampi(q, 0, n + 2)] : hdtib[clampi(q, 0, n + 2)];
+          const float hib = row[1 + st];  // 0.5 dt/dxb (stages 1, 2) or dt/dxb (stage 3) of cell q
           const float yn = fmaf(hib, Gp - G, cn[j][(u - 2 * (st + 1) - 1) & 15]);
           if constexpr (GEN)
             v = (q <= 0) ? gL[j] : ((q > n) ? gR[j] : yn);  // next stage input (position q)
@@ -412,7 +416,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       {
         const int q = t - 2 * NST - 1;
         const int cl = (u - 2 * NST - 1) & 15;
-        const float dtq = ftib[clampi(q, 0, n + 2)];
+        const float dtq = row[4];
         const float gq = gh[j][(u - 2 * NST - 1) & 7], gq1 = gh[j][(u - 2 * NST) & 7];
         uint32_t vo = vstore[j];
         if constexpr (GEN) {
