@@ -44,8 +44,8 @@ def parse():
                          "evaluation of the coarse-grid flux correction")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--tiling", default="",
-                    help="x_cells_per_lane,y_bins_per_group,z_bins_per_group: fixed tile shapes instead of "
-                         "the autotuner (profiling runs)")
+                    help="x_cells_per_lane,y_bins_per_group[/bins_per_thread],z_bins_per_group[/bins_per_thread]: "
+                         "fixed tile shapes instead of the autotuner (profiling runs)")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="learned workload: linear scale of the grids (1.0 = 512x512x40 training, "
                          "1024x768x80x20 evaluation)")
@@ -234,10 +234,13 @@ def main():
     g.set_problem(p)
     dt = g.time_step(0.0, 2, 0.9)
     if args.tiling:
-        tx, ty, tz = (int(v) for v in args.tiling.split(","))
+        sx, sy, sz = args.tiling.split(",")
+        tx = int(sx)
+        ty, by = (int(v) for v in (sy + "/0").split("/")[:2])
+        tz, bz = (int(v) for v in (sz + "/0").split("/")[:2])
         g.set_tiling(0, cells_per_thread=tx)
-        g.set_tiling(1, bins_per_group=ty)
-        g.set_tiling(2, bins_per_group=tz)
+        g.set_tiling(1, cells_per_thread=by, bins_per_group=ty)
+        g.set_tiling(2, cells_per_thread=bz, bins_per_group=tz)
         tiling = (tx, ty, tz)
     else:
         tiling, tune_ms = g.autotune(dt, 0.0, 3)
@@ -320,7 +323,8 @@ def main():
             "step_hbm_roofline_frac": round(step_alg_bytes * args.steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
             "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in ktimes.items()},
             "autotuned_tiling": {"x_cells_per_lane": tiling[0], "y_bins_per_group": tiling[1],
-                                 "z_bins_per_group": tiling[2]},
+                                 "z_bins_per_group": tiling[2], "y_bins_per_thread": g.tiling(1)[0],
+                                 "z_bins_per_thread": g.tiling(2)[0]},
             "mass_closure": abs(b["mass_cells"] + float(np.sum(b["ledger"])) - b["injected"]) / max(b["injected"],
                                                                                                      1e-300),
         },

@@ -48,7 +48,7 @@ EXPORTS = (
     "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
-    "fall3d_ads_set_graph", "fall3d_ads_autotune",
+    "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
     "fall3d_ads_ensemble_set_graph", "fall3d_ads_ensemble_synchronize", "fall3d_ads_ensemble_set_tiling",
@@ -111,6 +111,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_sweep_bytes": ([_vp, _int, _dp], _int),
         "fall3d_ads_set_graph": ([_vp, _int], _int),
         "fall3d_ads_autotune": ([_vp, _dbl, _dbl, _int, _vp, _vp], _int),
+        "fall3d_ads_get_tiling": ([_vp, _int, ctypes.POINTER(_int), ctypes.POINTER(_int)], _int),
         "fall3d_ads_ensemble_create": ([ctypes.POINTER(_vp), _int, ctypes.POINTER(_int), _int, _int, _int, _int,
                                         _int], _int),
         "fall3d_ads_ensemble_release": ([_vp], _int),
@@ -402,6 +403,13 @@ class ADS:
     def set_tiling(self, axis, cells_per_thread=0, columns=0, bins_per_group=0):
         self._chk(load_library().fall3d_ads_set_tiling(self._h, int(axis), int(cells_per_thread), int(columns),
                                                        int(bins_per_group)), "set_tiling")
+
+    def tiling(self, axis):
+        """(cells or bins per thread, bins per workgroup) in use for the sweep of axis."""
+        cpt, bpg = _int(), _int()
+        self._chk(load_library().fall3d_ads_get_tiling(self._h, int(axis), ctypes.byref(cpt), ctypes.byref(bpg)),
+                  "get_tiling")
+        return cpt.value, bpg.value
 
     def autotune(self, dt, stime=0.0, reps=3):
         """Time the candidate tile shapes of each sweep and keep the fastest

@@ -140,16 +140,20 @@ __global__ void k_dt_over_dxb(const double *__restrict__ dxb, int n3, double dt,
 
 __global__ void k_iter_table(const float *__restrict__ tab, const float *__restrict__ hdxp, int n, int nst,
                              float *__restrict__ itab) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= iter_rows(n)) return;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= iter_rows(n)) return;
+  const int t = row - kItabOffset;
   const int n3 = n + 3;
   auto cl = [&](int p) { return p < 0 ? 0 : (p > n + 2 ? n + 2 : p); };
   const float *dtib = tab, *hdtib = tab + n3, *ftib = tab + 2 * n3;
-  float *r = itab + (size_t)t * 8;
+  float *r = itab + (size_t)row * 8;
+  // stage factors are 0 outside the interior cells 1..n: a stage value at a
+  // ghost position then stays the frozen ghost
+  auto in = [&](int p) { return p >= 1 && p <= n; };
   r[0] = hdxp[cl(t - 2)];
-  r[1] = hdtib[cl(t - 3)];
-  r[2] = hdtib[cl(t - 5)];
-  r[3] = dtib[cl(t - 7)];
+  r[1] = in(t - 3) ? hdtib[t - 3] : 0.0f;
+  r[2] = in(t - 5) ? hdtib[t - 5] : 0.0f;
+  r[3] = in(t - 7) ? dtib[t - 7] : 0.0f;
   r[4] = ftib[cl(t - 2 * nst - 1)];
   r[5] = r[6] = r[7] = 0.0f;
 }

@@ -127,7 +127,7 @@ Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
     int bpg = c->user_bpg[dir] > 0 ? std::min(c->user_bpg[dir], 8) : 8;
     int groups = (c->nb + bpg - 1) / bpg;
     bpg = (c->nb + groups - 1) / groups;
-    t.cpt = (c->tm == 1) ? 2 : 1;  // bins per thread (two only for Euler; RK4 would spill)
+    t.cpt = c->user_cpt[dir] ? c->user_cpt[dir] : ((c->tm == 1) ? 2 : 1);  // bins per thread
     t.S = 1;
     t.TX = 64;
     t.bpg = bpg;
@@ -1025,25 +1025,30 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
   HIPCHK(c, hipEventCreate(&eb));
   const bool prof = c->prof;
   c->prof = false;
-  const int oc = c->user_cpt[0], ob[3] = {c->user_bpg[0], c->user_bpg[1], c->user_bpg[2]};
+  const int oc[3] = {c->user_cpt[0], c->user_cpt[1], c->user_cpt[2]};
+  const int ob[3] = {c->user_bpg[0], c->user_bpg[1], c->user_bpg[2]};
   double best[3] = {1e30, 1e30, 1e30};
   int pick[3] = {0, 0, 0};
   for (int d = 0; d < 3 && !rc; ++d) {
+    // x: cells per lane; y/z: bins per group g and bins per thread b as g + 16 (b - 1)
     std::vector<int> cand;
     if (d == 0) {
       for (int i = 0; i < kSweepCptCount; ++i)
         if (c->nx / kSweepCpt[i] + 1 <= 64) cand.push_back(kSweepCpt[i]);
     } else {
-      for (int g = 1; g <= std::min(c->nb, 8); ++g) {
-        const int groups = (c->nb + g - 1) / g;
-        if ((c->nb + groups - 1) / groups == g) cand.push_back(g);
-      }
+      for (int b = 1; b <= (c->comp ? 1 : 2); ++b)
+        for (int g = b; g <= std::min(c->nb, 8); ++g) {
+          const int groups = (c->nb + g - 1) / g;
+          if ((c->nb + groups - 1) / groups == g) cand.push_back(g + 16 * (b - 1));
+        }
     }
     for (int v : cand) {
-      if (d == 0)
+      if (d == 0) {
         c->user_cpt[0] = v;
-      else
-        c->user_bpg[d] = v;
+      } else {
+        c->user_bpg[d] = v & 15;
+        c->user_cpt[d] = (v >> 4) + 1;
+      }
       if ((rc = refresh_tiling(c))) break;
       const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
       const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
@@ -1071,10 +1076,12 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
         pick[d] = v;
       }
     }
-    if (d == 0)
-      c->user_cpt[0] = pick[0] ? pick[0] : oc;
-    else
-      c->user_bpg[d] = pick[d] ? pick[d] : ob[d];
+    if (d == 0) {
+      c->user_cpt[0] = pick[0] ? pick[0] : oc[0];
+    } else {
+      c->user_bpg[d] = pick[d] ? (pick[d] & 15) : ob[d];
+      c->user_cpt[d] = pick[d] ? (pick[d] >> 4) + 1 : oc[d];
+    }
   }
   c->prof = prof;
   (void)hipEventDestroy(ea);
@@ -1091,8 +1098,10 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
   for (int i = 0; i < 4; ++i)
     if (save[i]) (void)hipFree(save[i]);
   if (rc) {
-    c->user_cpt[0] = oc;
-    for (int d = 0; d < 3; ++d) c->user_bpg[d] = ob[d];
+    for (int d = 0; d < 3; ++d) {
+      c->user_cpt[d] = oc[d];
+      c->user_bpg[d] = ob[d];
+    }
     (void)refresh_tiling(c);
     return rc;
   }
@@ -1107,10 +1116,20 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
   return FALL3D_ADS_OK;
 }
 
+int fall3d_ads_get_tiling(fall3d_ads_ctx *c, int dir, int *cells_per_thread, int *bins_per_group) {
+  if (!c || dir < 0 || dir > 2) return FALL3D_ADS_EINVAL;
+  if (cells_per_thread) *cells_per_thread = c->tl[dir].cpt;
+  if (bins_per_group) *bins_per_group = c->tl[dir].bpg;
+  return FALL3D_ADS_OK;
+}
+
 int fall3d_ads_set_tiling(fall3d_ads_ctx *c, int dir, int cells_per_thread, int columns, int bins_per_group) {
   if (!c || dir < 0 || dir > 2) return FALL3D_ADS_EINVAL;
-  if (cells_per_thread && !sweep_cpt_supported(cells_per_thread))
+  if (dir == 0 && cells_per_thread && !sweep_cpt_supported(cells_per_thread))
     return fail(c, FALL3D_ADS_EINVAL, "unsupported cells per thread %d", cells_per_thread);
+  // y/z: bins per thread, 1 or 2 (2 is not built for the compensated state)
+  if (dir > 0 && (cells_per_thread < 0 || cells_per_thread > 2 || (cells_per_thread == 2 && c->comp)))
+    return fail(c, FALL3D_ADS_EINVAL, "unsupported bins per thread %d", cells_per_thread);
   if (columns && (columns < 1 || columns > 64)) return fail(c, FALL3D_ADS_EINVAL, "columns %d out of range", columns);
   const int oc = c->user_cpt[dir], ot = c->user_tx[dir], ob = c->user_bpg[dir];
   c->user_cpt[dir] = cells_per_thread;

@@ -131,9 +131,14 @@ int fail(fall3d_ads_ctx *c, int code, const char *fmt, ...);
 
 template <typename T>
 hipError_t dalloc(T **p, size_t n) {
+  // kDallocPad zeroed elements after the end: the x sweep reads a lane's
+  // cells with wide loads that may run up to 17 elements past a line's last
+  // face, which for the last line of an array is past its end
+  constexpr size_t kDallocPad = 64;
   *p = nullptr;
-  if (n == 0) n = 1;
-  return hipMalloc((void **)p, n * sizeof(T));
+  hipError_t e = hipMalloc((void **)p, (n + kDallocPad) * sizeof(T));
+  if (e == hipSuccess) e = hipMemset((void *)(*p + n), 0, kDallocPad * sizeof(T));
+  return e;
 }
 
 inline void dfree(void *p) {

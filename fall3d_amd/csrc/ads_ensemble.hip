@@ -38,6 +38,7 @@ struct fall3d_ads_ensemble {
 
   // common tile shape: cells per lane of the x sweep, bins per group of y/z
   int x_cpt = 0, bpg[3] = {0, 0, 0};
+  int bpt[3] = {0, 0, 0};  // y/z bins per thread (0: the members' default)
 
   // per axis: member arguments and workgroup prefix of the batched sweep
   SweepArgs *dargs[3] = {nullptr, nullptr, nullptr};
@@ -102,6 +103,8 @@ int apply_tiling(fall3d_ads_ensemble *e) {
     c->user_bpg[0] = c->nb;  // one bin group: the batch already has enough workgroups
     c->user_bpg[1] = e->bpg[1];
     c->user_bpg[2] = e->bpg[2];
+    c->user_cpt[1] = e->bpt[1];
+    c->user_cpt[2] = e->bpt[2];
     const int rc = refresh_tiling(c);
     if (rc) return mfail(e, m, rc);
   }
@@ -507,25 +510,33 @@ int fall3d_ads_ensemble_autotune(fall3d_ads_ensemble *e, const double *dt, const
     return rc;
   }
   // candidates: x cells per lane (every supported value that fits the
-  // widest member), y/z bins per group 1..min(nb, 8)
+  // widest member); y/z bins per group g in 1..min(nb, 8) and bins per
+  // thread b (2 not with the compensated state), encoded g + 16 (b - 1)
   int nxmax = 1;
-  for (auto *c : e->mem) nxmax = std::max(nxmax, c->nx);
+  bool comp = false;
+  for (auto *c : e->mem) {
+    nxmax = std::max(nxmax, c->nx);
+    comp = comp || c->comp;
+  }
   std::vector<int> xc, yz;
   for (int i = 0; i < kSweepCptCount; ++i)
     if (nxmax / kSweepCpt[i] + 1 <= 64) xc.push_back(kSweepCpt[i]);
-  for (int g = 1; g <= std::min(e->nb, 8); ++g) {
-    const int groups = (e->nb + g - 1) / g;
-    if ((e->nb + groups - 1) / groups == g) yz.push_back(g);  // distinct shapes only
-  }
+  for (int b = 1; b <= (comp ? 1 : 2); ++b)
+    for (int g = b; g <= std::min(e->nb, 8); ++g) {
+      const int groups = (e->nb + g - 1) / g;
+      if ((e->nb + groups - 1) / groups == g) yz.push_back(g + 16 * (b - 1));  // distinct shapes only
+    }
   double best[3] = {1e30, 1e30, 1e30};
-  int pick[3] = {e->x_cpt, e->bpg[1], e->bpg[2]};
+  int pick[3] = {e->x_cpt, e->bpg[1] + 16 * std::max(e->bpt[1] - 1, 0), e->bpg[2] + 16 * std::max(e->bpt[2] - 1, 0)};
   for (int d = 0; d < 3 && !rc; ++d) {
     const std::vector<int> &cand = (d == 0) ? xc : yz;
     for (int v : cand) {
-      if (d == 0)
+      if (d == 0) {
         e->x_cpt = v;
-      else
-        e->bpg[d] = v;
+      } else {
+        e->bpg[d] = v & 15;
+        e->bpt[d] = (v >> 4) + 1;
+      }
       if ((rc = apply_tiling(e))) break;
       if ((rc = build_tables(e, dt, st))) break;
       double ms = 0.0;
@@ -535,10 +546,12 @@ int fall3d_ads_ensemble_autotune(fall3d_ads_ensemble *e, const double *dt, const
         pick[d] = v;
       }
     }
-    if (d == 0)
+    if (d == 0) {
       e->x_cpt = pick[0];
-    else
-      e->bpg[d] = pick[d];
+    } else {
+      e->bpg[d] = pick[d] & 15;
+      e->bpt[d] = (pick[d] >> 4) + 1;
+    }
   }
   const int rr = snapshot(e, snap, true);
   free_snapshot(snap);

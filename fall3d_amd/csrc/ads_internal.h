@@ -65,11 +65,15 @@ struct SweepBatch {
 // [dt/dxb | 0.5 dt/dxb | w dt/dxb], w = 1/6 (RK4) or 1 (Euler, tm == 1).
 hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, hipStream_t st);
 // Per-iteration factors of the streaming sweep of an axis with n cells and
-// nst stages, rows t = 0 .. iter_rows(n) - 1 of 8 floats:
-//   0.5/dxp of face t-2, 0.5 dt/dxb of cells t-3 and t-5, dt/dxb of cell
-//   t-7, final factor of cell t-2 nst-1 (indices clamped to 0..n+2).
+// nst stages: row r (8 floats) belongs to iteration t = r - kItabOffset and
+// holds 0.5/dxp of face t-2, 0.5 dt/dxb of cells t-3 and t-5, dt/dxb of
+// cell t-7 (these three are 0 for cells outside 1..n) and the final factor
+// of cell t-2 nst-1 (indices clamped to 0..n+2). Iterations run from at
+// least 2 - kPF - 15 (> -kItabOffset) to n + 2 nst + 2 (the row fetched
+// one iteration ahead of the last).
 // tab: the dt_over_dxb table of the axis; hdxp: 0.5/dxp(f-1) at face f.
-__host__ __device__ inline int iter_rows(int n) { return n + 2 * 4 + 4; }
+constexpr int kItabOffset = 24;
+__host__ __device__ inline int iter_rows(int n) { return (n > 0 ? n : 0) + 2 * 4 + 3 + kItabOffset; }
 hipError_t iter_table(const float *tab, const float *hdxp, int n, int nst, float *itab, hipStream_t st);
 
 // Sweep launchers (ads_sweep_*.hip). cpt = cells per lane (x sweep) or

@@ -167,6 +167,16 @@ __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? l
 constexpr int kPF = F3D_STREAM_PREFETCH;
 static_assert(kPF >= 2 && kPF <= 6, "prefetch distance outside the ring capacity");
 
+
+// Calls f(integral_constant<int, U>) for U = U0 .. U1-1 (compile-time unrolled).
+template <int U0, int U1, class F>
+__device__ __forceinline__ void unroll_range(F &&f) {
+  if constexpr (U0 < U1) {
+    f(std::integral_constant<int, U0>{});
+    unroll_range<U0 + 1, U1>(f);
+  }
+}
+
 template <int DIR, int LIM, bool COMP, int NST, int BPT, bool FOUT = false>
 __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   // the wave index is uniform across the wave: say so, so that the per-wave
@@ -210,7 +220,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   typedef float f8 __attribute__((ext_vector_type(8)));
   typedef __attribute__((address_space(4))) const f8 cf8;
   cf8 *itab = (cf8 *)a.itab;
-  f8 rnext = itab[0];
+  f8 rnext;
 
   // lane byte offsets of position -1 (c), face 0 (winds), cell -1 (K); byte strides
   uint32_t vc, cs4, vv, vst4, vk, kst4;
@@ -256,7 +266,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       const float cnn = bload(rc[j], vc, cpos(n));
       gL[j] = (vfirst > 0.0f) ? (2.0f * 0.0f - c1) : c1;
       gR[j] = (vlast < 0.0f) ? (2.0f * 0.0f - cnn) : cnn;
-      const uint32_t vg = a.write_ghosts ? vstore[j] : kOOB;
+      const uint32_t vg = vstore[j];  // always written: the sweep reads them back
       bstore(rc[j], vg, cpos(-1), gL[j]);
       bstore(rc[j], vg, cpos(0), gL[j]);
       bstore(rc[j], vg, cpos(n + 1), gR[j]);
@@ -286,10 +296,6 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
 #pragma unroll
     for (int j = 0; j < BPT; ++j) gh[j][q] = lo[j][q] = 0.0f;
   }
-#pragma unroll
-  for (int j = 0; j < BPT; ++j)
-#pragma unroll
-    for (int q = 0; q < 16; ++q) cn[j][q] = 0.0f;
   // per-stage pipeline state: previous two inputs, last difference, last
   // slope, last face flux. Every stage starts as if it had consumed an
   // unbounded run of west ghosts (value gL: zero difference and slope).
@@ -301,36 +307,37 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       sa1[j][st] = sa2[j][st] = gL[j];
       sd[j][st] = sm[j][st] = sG[j][st] = 0.0f;
     }
-  // ring slots: cell p -> p & 15, face f -> f & 7. Preload what iterations
-  // t = -kPF..-1 would have fetched (cells -1..kPF-2, faces -2..kPF-3), so
-  // that the pipeline starts at t = 0.
+  // Ghosts come from the data, with no per-stage selects:
+  //  - every cell ring slot starts as the west ghost, and cells at positions
+  //    below 1 are either never fetched or fetched from the ghost cells the
+  //    code above has just written (clamped positions), so every stage sees
+  //    the west ghost there;
+  //  - cells past n are fetched from the east ghost cells written above;
+  //  - the stage factors of the iteration table are 0 outside 1..n, so a
+  //    stage's value at a ghost position stays the ring value: the frozen
+  //    ghosts of the reference.
+  // A wave reads its own ghost stores back in program order.
 #pragma unroll
-  for (int p = -1; p <= kPF - 2; ++p)
+  for (int j = 0; j < BPT; ++j)
 #pragma unroll
-    for (int j = 0; j < BPT; ++j) cn[j][p & 15] = bload(rc[j], vc, cpos(p));
-#pragma unroll
-  for (int f = -2; f <= kPF - 3; ++f) {
-    rw1[f & 7] = bload(rv1, vv, fpos(f));
-    rw2[f & 7] = bload(rv2, vv, fpos(f));
-    rkk[f & 7] = bload(rk, vk, kpos(f));
-    if constexpr (DIR == 2)
-#pragma unroll
-      for (int j = 0; j < BV; ++j) rws[j][f & 7] = bload(rvs[j], vv, fpos(f));
-  }
+    for (int q = 0; q < 16; ++q) cn[j][q] = gL[j];
 
   float RFb[BPT], RFe[BPT], tl1[BPT], trn[BPT];
 #pragma unroll
   for (int j = 0; j < BPT; ++j) RFb[j] = RFe[j] = tl1[j] = trn[j] = 0.0f;
   const int tend = n + 2 * NST + 1;  // iteration of the last final output (cell n)
+  // Whole blocks of 16 iterations that end at tend and start at or before
+  // 2 - kPF, the iteration that fetches cell 1 (earlier iterations only see
+  // ghosts and drop their stores).
+  const int nblk = (tend + kPF - 1 + 15) / 16;
+  const int tstart = tend + 1 - 16 * nblk;
+  rnext = itab[tstart + kItabOffset];
 
-  // One pipeline step at iteration t (u = t mod 16, a compile-time constant
-  // after unrolling). GEN: iteration that may touch a ghost, a boundary face
-  // or an out-of-range cell; otherwise every position is interior and the
-  // selects are compiled out.
-  auto step = [&](auto gen_tag, const int t, const int u) {
-    constexpr bool GEN = decltype(gen_tag)::value;
+  // One pipeline step at iteration t (u = (t - tstart) mod 16, a
+  // compile-time constant: ring slots are static registers).
+  auto step = [&](const int t, const int u) {
     const f8 row = rnext;  // [0.5/dxp(t-2), 0.5dt/dxb(t-3), 0.5dt/dxb(t-5), dt/dxb(t-7), final(t-2NST-1)]
-    rnext = itab[t + 1];
+    rnext = itab[t + 1 + kItabOffset];
     // ---- prefetch cell t+kPF-1 and face t+kPF-2
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
@@ -362,20 +369,14 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       kf[sl] = (rkk[(u - 3) & 7] + rkk[sl]) * row[0];
     }
     // ---- the pipeline: stage st consumes position t+1-2(st+1)
+    float Glast[BPT], Plast[BPT];
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
       const int jv = (DIR == 2) ? j : 0;
-      float v;
-      if constexpr (GEN) {
-        const int P = t - 1;
-        v = (P <= 0) ? gL[j] : ((P > n) ? gR[j] : cn[j][(u - 1) & 15]);
-      } else {
-        v = cn[j][(u - 1) & 15];
-      }
+      float v = cn[j][(u - 1) & 15];  // stage-0 input: position t-1
 #pragma unroll
       for (int st = 0; st < NST; ++st) {
-        const int f = t - 2 * (st + 1);  // face completed by this stage
-        const int fs = (u - 2 * (st + 1)) & 7;
+        const int fs = (u - 2 * (st + 1)) & 7;  // face t-2(st+1), completed by this stage
         const float d = v - sa1[j][st];
         const float m = limiter_slope<LIM>(sd[j][st], d);
         const float cL = fmaf(0.5f, sm[j][st], sa2[j][st]);
@@ -390,13 +391,10 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
           gh[j][fs] = gh[j][fs] + G;
         else
           gh[j][fs] = fmaf(2.0f, G, gh[j][fs]);
-        if constexpr (GEN) {
-          if (st == NST - 1) {  // reference end-face fluxes F + P of the last stage
-            RFb[j] = (f == 1) ? fmaf(2.0f, Pd, G) : RFb[j];
-            RFe[j] = (f == n + 1) ? fmaf(2.0f, Pd, G) : RFe[j];
-          }
+        if (st == NST - 1) {
+          Glast[j] = G;
+          Plast[j] = Pd;
         }
-        const int q = f - 1;  // output cell of this stage
         const float Gp = sG[j][st];
         sa2[j][st] = sa1[j][st];
         sa1[j][st] = v;
@@ -404,26 +402,19 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
         sm[j][st] = m;
         sG[j][st] = G;
         if (st < NST - 1) {
-          const float hib = row[1 + st];  // 0.5 dt/dxb (stages 1, 2) or dt/dxb (stage 3) of cell q
-          const float yn = fmaf(hib, Gp - G, cn[j][(u - 2 * (st + 1) - 1) & 15]);
-          if constexpr (GEN)
-            v = (q <= 0) ? gL[j] : ((q > n) ? gR[j] : yn);  // next stage input (position q)
-          else
-            v = yn;
+          // next stage input (position t-2st-3); the factor (0.5 dt/dxb for
+          // stages 1, 2, dt/dxb for stage 3) is 0 at ghost positions
+          const float hib = row[1 + st];
+          v = fmaf(hib, Gp - G, cn[j][(u - 2 * (st + 1) - 1) & 15]);
         }
       }
-      // ---- final update of cell q = t - 2 NST - 1
+      // ---- final update of cell q = t - 2 NST - 1 (stores before cell 1 dropped)
       {
         const int q = t - 2 * NST - 1;
         const int cl = (u - 2 * NST - 1) & 15;
         const float dtq = row[4];
         const float gq = gh[j][(u - 2 * NST - 1) & 7], gq1 = gh[j][(u - 2 * NST) & 7];
-        uint32_t vo = vstore[j];
-        if constexpr (GEN) {
-          vo = (q >= 1 && q <= n) ? vstore[j] : kOOB;
-          tl1[j] = (q == 1) ? dtq * gq : tl1[j];
-          trn[j] = (q == n) ? dtq * gq1 : trn[j];
-        }
+        const uint32_t vo = (q >= 1) ? vstore[j] : kOOB;
         if constexpr (COMP) {
           float hv = cn[j][cl], lv = lo[j][(u - 2 * NST - 1) & 7];
           comp_add(hv, lv, dtq * gq);
@@ -435,11 +426,29 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
         }
         if constexpr (FOUT) {  // face mass flux per unit area: faces q (1..n) and n+1
           const uint32_t fo = fpos(q), fo1 = fpos(q + 1);
-          const uint32_t vf = (vo != kOOB) ? vv : kOOB;  // same predicate, face-array lane offset
+          const uint32_t vf = (q >= 1 && vstore[j] != kOOB) ? vv : kOOB;
           const bool add = a.fout_add != 0;
           bstore(rf[j], vf, fo, fmaf(a.fscale, gq, add ? bload(rf[j], vv, fo) : 0.0f));
           const uint32_t ve = (q == n) ? vf : kOOB;
           bstore(rf[j], ve, fo1, fmaf(a.fscale, gq1, add ? bload(rf[j], vv, fo1) : 0.0f));
+        }
+      }
+    }
+    // ---- boundary captures (three iterations per line; a branch, not selects):
+    // the reference's end-face fluxes F + P of the last stage at faces 1 and
+    // n+1, and the final-update fluxes of cells 1 and n
+    if (t == 2 * NST + 1 || t == 2 * NST + 2 || t == tend) {
+      __asm__ volatile("");
+      const float dtq = row[4];
+#pragma unroll
+      for (int j = 0; j < BPT; ++j) {
+        const float rf = fmaf(2.0f, Plast[j], Glast[j]);
+        const float gq = gh[j][(u - 2 * NST - 1) & 7], gq1 = gh[j][(u - 2 * NST) & 7];
+        if (t == 2 * NST + 1) RFb[j] = rf;
+        if (t == 2 * NST + 2) tl1[j] = dtq * gq;
+        if (t == tend) {
+          RFe[j] = rf;
+          trn[j] = dtq * gq1;
         }
       }
     }
@@ -448,21 +457,8 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
     __builtin_amdgcn_sched_barrier(0);
   };
 
-  // Blocks of 16 iterations. Interior blocks (every t in [2 NST + 3, n + 1]:
-  // no ghost input, no boundary face, no capture, every output in range) run
-  // without selects; the others stop after iteration tend.
-  for (int t0 = 0; t0 <= tend; t0 += 16) {
-    if (t0 >= 2 * NST + 3 && t0 + 15 <= n + 1) {
-#pragma unroll
-      for (int u = 0; u < 16; ++u) step(std::false_type{}, t0 + u, u);
-    } else {
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        if (t0 + u > tend) break;
-        step(std::true_type{}, t0 + u, u);
-      }
-    }
-  }
+  for (int t0 = tstart; t0 <= tend; t0 += 16)
+    unroll_range<0, 16>([&](auto U) { step(t0 + decltype(U)::value, decltype(U)::value); });
 
   double rb = 0.0, re = 0.0, lb = 0.0, le = 0.0;
   if (lane_ok) {
@@ -513,6 +509,43 @@ __global__ void __launch_bounds__(512) k_stream_batch(SweepBatch b) {
 // k_sweep_x: x sweep, one wave per line, C cells per lane
 // =============================================================================
 
+// C consecutive floats of one lane in as few wide accesses as possible
+// (4-, 2- and 1-float pieces). The texture unit spends about the same
+// cycles on a lane's 16-byte access as on a 4-byte one, so wide accesses
+// cut its load per byte (addresses need only 4-byte alignment here).
+typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f2a __attribute__((ext_vector_type(2), aligned(4)));
+template <int C, int Q = 0>
+__device__ __forceinline__ void load_cells(const float *p, float (&v)[C]) {
+  if constexpr (C - Q >= 4) {
+    const f4a t = *(const f4a *)(p + Q);
+    v[Q] = t.x;
+    v[Q + 1] = t.y;
+    v[Q + 2] = t.z;
+    v[Q + 3] = t.w;
+    load_cells<C, Q + 4>(p, v);
+  } else if constexpr (C - Q >= 2) {
+    const f2a t = *(const f2a *)(p + Q);
+    v[Q] = t.x;
+    v[Q + 1] = t.y;
+    load_cells<C, Q + 2>(p, v);
+  } else if constexpr (C - Q == 1) {
+    v[Q] = p[Q];
+  }
+}
+template <int C, int Q = 0>
+__device__ __forceinline__ void store_cells(float *p, const float (&v)[C]) {
+  if constexpr (C - Q >= 4) {
+    *(f4a *)(p + Q) = f4a{v[Q], v[Q + 1], v[Q + 2], v[Q + 3]};
+    store_cells<C, Q + 4>(p, v);
+  } else if constexpr (C - Q >= 2) {
+    *(f2a *)(p + Q) = f2a{v[Q], v[Q + 1]};
+    store_cells<C, Q + 2>(p, v);
+  } else if constexpr (C - Q == 1) {
+    p[Q] = v[Q];
+  }
+}
+
 template <int C, int LIM, bool COMP, int NST, bool FOUT = false>
 __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, const int by) {
   static_assert(C >= 2, "segments need two edge cells");
@@ -541,36 +574,36 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   const float s = a.stime, s1 = 1.0f - a.stime;
   const float dt = a.dt;
 
-  // wind and diffusion of this line, read once for all bins
+  // wind and diffusion of this line, read once for all bins, with wide
+  // unconditional loads (idle lanes read line (1, 1)). Faces past n+1 only
+  // feed ghost positions, whose stage multipliers are zero, so their values
+  // (the next line's, or the arrays' zeroed padding) do not matter.
   float up[C], um[C], kf[C], hb[C], db[C], fb[C];
   float vbeg = 0.0f, vend = 0.0f;
   {
-    float kprev = act ? a.kd[kbase + (size_t)s0] : 0.0f;  // cell s0-1 (stored at index s0)
+    float w1v[C], w2v[C], kk[C + 1], hx[C];
+    load_cells<C>(a.vel1 + vbase + s0, w1v);
+    load_cells<C>(a.vel2 + vbase + s0, w2v);
+    load_cells<C + 1>(a.kd + kbase + s0, kk);  // cells s0-1 .. s0+C-1 (cell p stored at index p+1)
+    load_cells<C>(a.hdxp + s0, hx);
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      int f = s0 + q;
-      f = f > n + 2 ? n + 2 : f;
-      float v = 0.0f, kc = 0.0f, h = 0.0f;
-      if (act) {
-        v = s1 * a.vel1[vbase + f] + s * a.vel2[vbase + f];
-        kc = a.kd[kbase + (size_t)(f + 1)];
-        h = a.hdxp[f];
-      }
+      const float v = s1 * w1v[q] + s * w2v[q];
       up[q] = fmaxf(v, 0.0f);
       um[q] = fminf(v, 0.0f);
-      kf[q] = (kprev + kc) * h;
-      kprev = kc;
+      kf[q] = (kk[q] + kk[q + 1]) * hx[q];
     }
     // stage multipliers are zero beyond cell n: the ghost positions of the
     // last segment then keep their frozen value through every stage
+    load_cells<C>(a.dtib + s0, db);
+    load_cells<C>(a.hdtib + s0, hb);
+    load_cells<C>(a.ftib + s0, fb);
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      int p = s0 + q;
-      const bool in = act && p <= n;
-      p = p > n + 2 ? n + 2 : p;
-      db[q] = in ? a.dtib[p] : 0.0f;
-      hb[q] = in ? a.hdtib[p] : 0.0f;
-      fb[q] = in ? a.ftib[p] : 0.0f;
+      const bool in = act && s0 + q <= n;
+      db[q] = in ? db[q] : 0.0f;
+      hb[q] = in ? hb[q] : 0.0f;
+      fb[q] = in ? fb[q] : 0.0f;
     }
     if (act) {
       vbeg = s1 * a.vel1[vbase + 1] + s * a.vel2[vbase + 1];
@@ -582,18 +615,14 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   const int bfirst = a.bin0 + by * a.nbin;
   const int blast = min(bfirst + a.nbin, a.nbins);
 
-  // prefetch the first bin
+  // prefetch the first bin. Loads are unconditional and wide: a lane's C
+  // cells may run past n (the ghosts and the next row's first cells, still
+  // inside the array; replaced by the east ghost below), and idle lanes read
+  // line (1, 1).
   float nxt[C], nlo[C];
-#pragma unroll
-  for (int q = 0; q < C; ++q) {
-    const int p = s0 + q;
-    nxt[q] = 0.0f;
-    nlo[q] = 0.0f;
-    if (act && p <= n && bfirst < blast) {
-      nxt[q] = a.chi[cbase + cbin * bfirst + p];
-      if constexpr (COMP) nlo[q] = a.clo[cbase + cbin * bfirst + p];
-    }
-  }
+  const int bpre = (bfirst < blast) ? bfirst : 0;
+  load_cells<C>(a.chi + cbase + cbin * bpre + s0, nxt);
+  if constexpr (COMP) load_cells<C>(a.clo + cbase + cbin * bpre + s0, nlo);
 
   for (int b = bfirst; b < blast; ++b) {
     const size_t cb = cbase + cbin * (size_t)b;
@@ -605,14 +634,8 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
     }
     // prefetch the next bin while this one is integrated
     if (b + 1 < blast) {
-#pragma unroll
-      for (int q = 0; q < C; ++q) {
-        const int p = s0 + q;
-        if (act && p <= n) {
-          nxt[q] = a.chi[cb + cbin + p];
-          if constexpr (COMP) nlo[q] = a.clo[cb + cbin + p];
-        }
-      }
+      load_cells<C>(a.chi + cb + cbin + s0, nxt);
+      if constexpr (COMP) load_cells<C>(a.clo + cb + cbin + s0, nlo);
     }
     // free-flow ghosts (Sources/mod_ADS.F90:1239-1265), frozen for the sweep.
     // Cell n is in the last segment, or (when that segment starts at n+1)
@@ -626,7 +649,10 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       for (int q = 1; q < C; ++q) cl = (q == qn) ? cn[q] : cl;
       gR = (vend < 0.0f) ? (2.0f * 0.0f - cl) : cl;
 #pragma unroll
-      for (int q = 0; q < C; ++q) cn[q] = (q > qn) ? gR : cn[q];
+      for (int q = 0; q < C; ++q) {
+        cn[q] = (q > qn) ? gR : cn[q];
+        lo[q] = (q > qn) ? 0.0f : lo[q];  // ghost low words are 0
+      }
     }
     if (act && a.write_ghosts) {
       if (seg == 0) {
@@ -719,23 +745,38 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
         if (act && p <= n + 1) fo[p] = fmaf(a.fscale, gh[q], a.fout_add ? fo[p] : 0.0f);
       }
     }
+    // Final update. Beyond cell n the factor fb is 0 and cn holds the east
+    // ghost, so a lane whose cells end at n+2 or earlier stores them all
+    // with wide stores (cells n+1, n+2 get the ghost value they hold); a
+    // lane reaching further (the last segment) stores cells up to n only.
     float tl0 = 0.0f, trn = 0.0f;
+    float hv[C], lv[C];
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      const int p = s0 + q;
       const float tl = fb[q] * gh[q];
       const float tr = fb[q] * gh[q + 1];
       if (q == 0) tl0 = tl;
       if (q == qn) trn = tr;
-      if (act && p <= n) {
-        if constexpr (COMP) {
-          float hv = cn[q], lv = lo[q];
-          comp_add(hv, lv, tl);
-          comp_add(hv, lv, -tr);
-          a.chi[cb + p] = hv;
-          a.clo[cb + p] = lv;
-        } else {
-          a.chi[cb + p] = fmaf(fb[q], gh[q] - gh[q + 1], cn[q]);
+      if constexpr (COMP) {
+        hv[q] = cn[q];
+        lv[q] = lo[q];
+        comp_add(hv[q], lv[q], tl);
+        comp_add(hv[q], lv[q], -tr);
+      } else {
+        hv[q] = fmaf(fb[q], gh[q] - gh[q + 1], cn[q]);
+      }
+    }
+    if (act) {
+      if (s0 + C - 1 <= n + 2) {
+        store_cells<C>(a.chi + cb + s0, hv);
+        if constexpr (COMP) store_cells<C>(a.clo + cb + s0, lv);
+      } else {
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+          if (s0 + q <= n) {
+            a.chi[cb + s0 + q] = hv[q];
+            if constexpr (COMP) a.clo[cb + s0 + q] = lv[q];
+          }
         }
       }
     }

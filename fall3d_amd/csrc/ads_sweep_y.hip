@@ -4,15 +4,16 @@
 
 namespace f3d {
 
-// Two bins per thread only for Euler: the RK4 pipeline of two bins does not
-// fit in 256 VGPRs.
+// Two bins per thread: Euler, and RK4 in the plain FP32 state (the
+// compensated RK4 pipeline of two bins needs too many VGPRs).
 #define F3D_STREAM_CASES(LAUNCH)                                                       \
   LAUNCH(1, false, 4, 1) LAUNCH(2, false, 4, 1) LAUNCH(3, false, 4, 1)                 \
   LAUNCH(1, true, 4, 1) LAUNCH(2, true, 4, 1) LAUNCH(3, true, 4, 1)                    \
   LAUNCH(1, false, 1, 1) LAUNCH(2, false, 1, 1) LAUNCH(3, false, 1, 1)                 \
   LAUNCH(1, true, 1, 1) LAUNCH(2, true, 1, 1) LAUNCH(3, true, 1, 1)                    \
   LAUNCH(1, false, 1, 2) LAUNCH(2, false, 1, 2) LAUNCH(3, false, 1, 2)                 \
-  LAUNCH(1, true, 1, 2) LAUNCH(2, true, 1, 2) LAUNCH(3, true, 1, 2)
+  LAUNCH(1, true, 1, 2) LAUNCH(2, true, 1, 2) LAUNCH(3, true, 1, 2)                    \
+  LAUNCH(1, false, 4, 2) LAUNCH(2, false, 4, 2) LAUNCH(3, false, 4, 2)
 
 hipError_t launch_sweep_y(int bpt, int limiter, bool comp, int nst, const SweepArgs &a, dim3 grid,
                           dim3 block, hipStream_t st) {

@@ -184,13 +184,19 @@ int fall3d_ads_get_budget(fall3d_ads_ctx *ctx, double out[FALL3D_ADS_BUDGET_LEN]
 int fall3d_ads_set_stream(fall3d_ads_ctx *ctx, void *stream);
 int fall3d_ads_synchronize(fall3d_ads_ctx *ctx);
 
-/* Tuning: cells per thread (0 = automatic) and x-columns per workgroup for
- * the sweep of axis dir (0 x, 1 y, 2 z); bins per workgroup. */
+/* Tuning of the sweep of axis dir (0 x, 1 y, 2 z), 0 = automatic:
+ * cells_per_thread is the x sweep's cells per lane, and the y/z sweeps' bins
+ * per thread (1 or 2; 2 not with the compensated state); columns: x-columns
+ * per workgroup; bins_per_group: bins per workgroup. */
 int fall3d_ads_set_tiling(fall3d_ads_ctx *ctx, int dir, int cells_per_thread, int columns, int bins_per_group);
+/* The tile shape in use for axis dir: cells (x) or bins (y/z) per thread,
+ * bins per workgroup. */
+int fall3d_ads_get_tiling(fall3d_ads_ctx *ctx, int dir, int *cells_per_thread, int *bins_per_group);
 /* Autotuner: times every candidate tile shape of each sweep (x: cells per
- * lane; y, z: bins per workgroup) on the context's fields, keeps the
- * fastest and restores state, deposit and accumulators. tiling (3 ints) and
- * best_ms (3 doubles) may be NULL. */
+ * lane; y, z: bins per workgroup and bins per thread) on the context's
+ * fields, keeps the fastest and restores state, deposit and accumulators.
+ * tiling (3 ints: x cells per lane, y and z bins per workgroup; the rest via
+ * fall3d_ads_get_tiling) and best_ms (3 doubles) may be NULL. */
 int fall3d_ads_autotune(fall3d_ads_ctx *ctx, double dt, double stime, int reps, int tiling[3], double best_ms[3]);
 
 /* Per-kernel timing with HIP events on the context stream.

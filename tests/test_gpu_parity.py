@@ -232,6 +232,39 @@ def test_deterministic_and_tiling_invariant():
         np.testing.assert_allclose(led, outs[0][1], rtol=1e-12, atol=1e-12 * np.abs(outs[0][1]).max())
 
 
+@pytest.mark.parametrize("dims", [(40, 36, 20, 3), (70, 20, 3, 4), (33, 9, 2, 5), (65, 4, 2, 2)])
+def test_stream_bins_per_thread_invariant(dims):
+    """y/z streaming sweeps with two bins per thread give the concentration
+    and deposit bits of one bin per thread (RK4 and Euler), also for lines
+    shorter than the stage pipeline (nz = 2, 3), odd bin counts and
+    partial 64-column tiles; the RK4 result matches the FP64 oracle."""
+    print(f"case dims={dims}", flush=True)
+    nx, ny, nz, nb = dims
+    p = operational_problem(nx, ny, nz, nb, seed=sum(dims))
+    for tm in ("rk", "euler"):
+        outs = []
+        for bpt in (1, 2):
+            g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", tm, 8, "fp32")
+            g.set_problem(p)
+            for ax in (1, 2):
+                g.set_tiling(ax, cells_per_thread=bpt, bins_per_group=2 if nb > 1 else 1)
+            assert g.tiling(1)[0] == bpt and g.tiling(2)[0] == bpt
+            dt = g.time_step(0.0, 2, 0.9)
+            g.run(dt, 5, 0)
+            outs.append((g.get_state(), g.budget()["ledger"], dt))
+            g.release()
+        (c1, a1), l1, dt = outs[0]
+        (c2, a2), l2, _ = outs[1]
+        assert np.array_equal(c1, c2) and np.array_equal(a1, a2), tm
+        np.testing.assert_allclose(l2, l1, rtol=1e-12, atol=1e-12 * max(np.abs(l1).max(), 1e-300))
+        if tm == "rk":
+            o = Oracle(p, "minmod", "rk")
+            cr, ar = p.new_c(), p.new_acum()
+            for it in range(5):
+                o.step(dt, it, cr, ar)
+            check_field(c2[2:-2, 2:-2, 2:-2], cr[2:-2, 2:-2, 2:-2], f"c two bins per thread {dims}")
+
+
 def test_operational_full_size_two_steps():
     """360x240x60, 10 bins (the benchmark workload) for two steps vs the
     FP64 oracle, plus budget closure of the compensated state."""
