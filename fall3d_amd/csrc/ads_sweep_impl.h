@@ -160,12 +160,23 @@ __device__ __forceinline__ void bstore(rsrc_t r, uint32_t voff, uint32_t soff, f
 }
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-#ifndef F3D_STREAM_PREFETCH
-#define F3D_STREAM_PREFETCH 4
+// prefetch distance (cells) of the streaming sweeps: rows arrive in groups
+// of 4 iterations; kGD groups are in flight (fetched kGD groups ahead)
+constexpr int kPF = 4;
+#ifndef F3D_STREAM_GROUPS
+#define F3D_STREAM_GROUPS 2
 #endif
-// prefetch distance (cells); the 8-deep face rings need kPF <= 6
-constexpr int kPF = F3D_STREAM_PREFETCH;
-static_assert(kPF >= 2 && kPF <= 6, "prefetch distance outside the ring capacity");
+constexpr int kGD = F3D_STREAM_GROUPS;
+static_assert(kGD == 1 || kGD == 2 || kGD == 4, "groups in flight must divide the 4 groups of a block");
+
+// dynamic LDS of a streaming-sweep workgroup, per wave: one 4 x 64-float
+// tile per fetched array (BPT cell arrays, w1, w2, K and BPT v_s arrays in
+// z), one per bin for the outputs (stored 4 rows at once), and per bin the
+// four boundary captures and two ghost values of every lane
+template <int DIR, int BPT>
+constexpr size_t stream_lds_bytes(unsigned threads) {
+  return (size_t)(threads / 64) * ((size_t)(2 * BPT + 3 + ((DIR == 2) ? BPT : 0)) * 1024 + (size_t)BPT * 6 * 256);
+}
 
 
 // Calls f(integral_constant<int, U>) for U = U0 .. U1-1 (compile-time unrolled).
@@ -239,6 +250,22 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
     vk = vv;
     kst4 = vst4;
   }
+  // grouped row fetches: lane L loads the 4 columns 4(L%16)..+3 of the
+  // wave's tile from row L/16 of a 4-row group (lanes past nx read column 1)
+  const int lq = lane & 15, lg = lane >> 4;
+  const int col4 = tile * 64 + 1 + 4 * lq;
+  const int c4 = (col4 <= nx) ? col4 : 1;
+  uint32_t xc, xv, xk;
+  if constexpr (DIR == 1) {
+    xc = (uint32_t)(4 * ((size_t)(c4 + 1) + (size_t)(nx + 4) * ((size_t)(ny + 4) * (t2 + 1))));
+    xv = (uint32_t)(4 * ((size_t)(c4 - 1) + (size_t)nx * (ny + 3) * (t2 - 1)));
+    xk = (uint32_t)(4 * ((size_t)(c4 - 1) + (size_t)nx * (ny + 4) * (t2 - 1)));
+  } else {
+    xc = (uint32_t)(4 * ((size_t)(c4 + 1) + (size_t)(nx + 4) * (t2 + 1)));
+    xv = (uint32_t)(4 * ((size_t)(c4 - 1) + (size_t)nx * (t2 - 1)));
+    xk = xv;
+  }
+  const uint32_t xcl = xc + (uint32_t)lg * cs4, xvl = xv + (uint32_t)lg * vst4, xkl = xk + (uint32_t)lg * kst4;
   const bool lane_ok = (i <= nx);
   uint32_t vstore[BPT];
 #pragma unroll
@@ -296,35 +323,95 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
 #pragma unroll
     for (int j = 0; j < BPT; ++j) gh[j][q] = lo[j][q] = 0.0f;
   }
-  // per-stage pipeline state: previous two inputs, last difference, last
-  // slope, last face flux. Every stage starts as if it had consumed an
-  // unbounded run of west ghosts (value gL: zero difference and slope).
-  float sa1[BPT][NST], sa2[BPT][NST], sd[BPT][NST], sm[BPT][NST], sG[BPT][NST];
+  // per-stage pipeline state: previous input, last difference, left state
+  // of the next face (previous input plus half its slope), last face flux.
+  // Every stage starts as if it had consumed an unbounded run of west
+  // ghosts (value gL: zero difference and slope).
+  float sa1[BPT][NST], sd[BPT][NST], scl[BPT][NST], sG[BPT][NST];
 #pragma unroll
   for (int j = 0; j < BPT; ++j)
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
-      sa1[j][st] = sa2[j][st] = gL[j];
-      sd[j][st] = sm[j][st] = sG[j][st] = 0.0f;
+      sa1[j][st] = scl[j][st] = gL[j];
+      sd[j][st] = sG[j][st] = 0.0f;
     }
+  // Row groups. Every 4 iterations a wave fetches the next 4 rows of each
+  // array with one 16-byte-per-lane load (a quarter of the texture-unit
+  // work of 4 single-float loads) and transposes them through its own LDS
+  // tile (no barrier: one wave writes and reads it, in order). Arrays: c of
+  // each bin (cells), w1, w2 (faces), K (cells), v_s of each bin (z, faces).
+  // Row offsets sit in the VGPR offset, which the buffer range check covers.
+  //
   // Ghosts come from the data, with no per-stage selects:
-  //  - every cell ring slot starts as the west ghost, and cells at positions
-  //    below 1 are either never fetched or fetched from the ghost cells the
-  //    code above has just written (clamped positions), so every stage sees
-  //    the west ghost there;
-  //  - cells past n are fetched from the east ghost cells written above;
+  //  - cell ring slots start as the west ghost; a fetched cell at a
+  //    position below 1 becomes the west ghost, one past n the east ghost
+  //    (a select only in the groups that touch a line end);
   //  - the stage factors of the iteration table are 0 outside 1..n, so a
   //    stage's value at a ghost position stays the ring value: the frozen
   //    ghosts of the reference.
-  // A wave reads its own ghost stores back in program order.
+  constexpr int NAC = BPT, NAW = 3 + ((DIR == 2) ? BPT : 0);  // cell arrays, face arrays (w1, w2, K, v_s)
+  constexpr int NA = NAC + NAW;
+  extern __shared__ float4 stream_lds[];
+  // per wave: NA + BPT tiles of 4 x 64 floats, then 6 floats per bin and lane
+  constexpr int kWaveFloats = (NA + BPT) * 256 + 6 * BPT * 64;
+  static_assert(kWaveFloats * 4 == (int)stream_lds_bytes<DIR, BPT>(64), "LDS layout and launch size disagree");
+  float4 *tile_lds = (float4 *)((float *)stream_lds + (size_t)wave * kWaveFloats);
+  const float *tile_f = (const float *)tile_lds;
+  float *out_f = (float *)(tile_lds + NA * 64);  // output tiles: bin j, row r, column lane
+  // output rows of a group are stored by lane L for columns 4(L%16)..+3 of
+  // row L/16: whole 16-byte pieces inside 1..nx, single floats where the
+  // piece crosses nx (then only columns up to nx)
+  bool sfull[BPT], spart[BPT];
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    sfull[j] = bok[j] && (col4 + 3 <= nx);
+    spart[j] = bok[j] && (col4 <= nx) && (col4 + 3 > nx);
+  }
+  float4 fc[kGD][NAC], fw[kGD][NAW];  // rows in flight, one set per group ahead
+#pragma unroll
+  for (int k = 0; k < kGD; ++k) {
+#pragma unroll
+    for (int q = 0; q < NAC; ++q) fc[k][q] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int q = 0; q < NAW; ++q) fw[k][q] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+  // fetch into set k the rows of the group that starts at iteration tg + 4:
+  // cells tg+3..tg+6, faces tg+2..tg+5, K of cells tg+2..tg+5
+  auto fetch = [&](const int tg, auto K) {
+    constexpr int k = decltype(K)::value;
+    // uniform row offset (scalar multiply) plus the lane's constant part;
+    // rows outside the line wrap or run past the array and come back as
+    // other lines' values or 0 from the range check: only ghost positions
+    // and faces outside 1..n+1 see them
+    const uint32_t oc = xcl + (uint32_t)((tg + 4) * (int)cs4);
+    const uint32_t ov = xvl + (uint32_t)((tg + 2) * (int)vst4);
+    const uint32_t ok = xkl + (uint32_t)((tg + 3) * (int)kst4);
+#pragma unroll
+    for (int j = 0; j < BPT; ++j)
+      fc[k][j] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rc[j], (int)oc, 0, 0));
+    fw[k][0] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rv1, (int)ov, 0, 0));
+    fw[k][1] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rv2, (int)ov, 0, 0));
+    fw[k][2] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rk, (int)ok, 0, 0));
+    if constexpr (DIR == 2)
+#pragma unroll
+      for (int j = 0; j < BPT; ++j)
+        fw[k][3 + j] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rvs[j], (int)ov, 0, 0));
+  };
 #pragma unroll
   for (int j = 0; j < BPT; ++j)
 #pragma unroll
     for (int q = 0; q < 16; ++q) cn[j][q] = gL[j];
 
-  float RFb[BPT], RFe[BPT], tl1[BPT], trn[BPT];
+  // boundary captures (RFb, tl1, RFe, trn of each bin) live in LDS until the
+  // end: each is written once per line; so do the ghost values (read only
+  // in the groups at the line ends)
+  float *cap_f = out_f + BPT * 4 * 64;  // [4][BPT][64]
+  float *ghost_f = cap_f + 4 * BPT * 64;  // [2][BPT][64]
 #pragma unroll
-  for (int j = 0; j < BPT; ++j) RFb[j] = RFe[j] = tl1[j] = trn[j] = 0.0f;
+  for (int j = 0; j < BPT; ++j) {
+    ghost_f[(0 * BPT + j) * 64 + lane] = gL[j];
+    ghost_f[(1 * BPT + j) * 64 + lane] = gR[j];
+  }
   const int tend = n + 2 * NST + 1;  // iteration of the last final output (cell n)
   // Whole blocks of 16 iterations that end at tend and start at or before
   // 2 - kPF, the iteration that fetches cell 1 (earlier iterations only see
@@ -332,29 +419,57 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   const int nblk = (tend + kPF - 1 + 15) / 16;
   const int tstart = tend + 1 - 16 * nblk;
   rnext = itab[tstart + kItabOffset];
+  // the first group's rows are all ghosts (the zeros above become them);
+  // the rows of the next kGD-1 groups are fetched here
+  unroll_range<1, kGD>([&](auto K) { fetch(tstart + 4 * (decltype(K)::value - 1), K); });
 
   // One pipeline step at iteration t (u = (t - tstart) mod 16, a
   // compile-time constant: ring slots are static registers).
-  auto step = [&](const int t, const int u) {
+  auto step = [&](const int t, auto U) {
+    constexpr int u = decltype(U)::value;
     const f8 row = rnext;  // [0.5/dxp(t-2), 0.5dt/dxb(t-3), 0.5dt/dxb(t-5), dt/dxb(t-7), final(t-2NST-1)]
     rnext = itab[t + 1 + kItabOffset];
-    // ---- prefetch cell t+kPF-1 and face t+kPF-2
+    if constexpr ((u & 3) == 0) {
+      // ---- group start: the rows fetched kGD groups ago (cells t-1..t+2,
+      // faces and K cells t-2..t+1) into the rings, then the fetch for the
+      // group kGD ahead into the same set
+      constexpr int k = (u >> 2) % kGD;
 #pragma unroll
-    for (int j = 0; j < BPT; ++j) {
-      cn[j][(u + kPF - 1) & 15] = bload(rc[j], vc, cpos(t + kPF - 1));
-      if constexpr (COMP) lo[j][(u + kPF - 2 * NST - 1) & 7] = bload(rl[j], vc, cpos(t + kPF - 2 * NST - 1));
-    }
-    {
-      const int f = t + kPF - 2;
-      const int sl = (u + kPF - 2) & 7;
-      const uint32_t fo = fpos(f);
-      rw1[sl] = bload(rv1, vv, fo);
-      rw2[sl] = bload(rv2, vv, fo);
-      rkk[sl] = bload(rk, vk, kpos(f));
-      if constexpr (DIR == 2)
+      for (int q = 0; q < NAC; ++q) tile_lds[q * 64 + lane] = fc[k][q];
 #pragma unroll
-        for (int j = 0; j < BV; ++j) rws[j][sl] = bload(rvs[j], vv, fo);
+      for (int q = 0; q < NAW; ++q) tile_lds[(NAC + q) * 64 + lane] = fw[k][q];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int j = 0; j < BPT; ++j) cn[j][(u - 1 + r) & 15] = tile_f[(j * 4 + r) * 64 + lane];
+        rw1[(u - 2 + r) & 7] = tile_f[((NAC + 0) * 4 + r) * 64 + lane];
+        rw2[(u - 2 + r) & 7] = tile_f[((NAC + 1) * 4 + r) * 64 + lane];
+        rkk[(u - 2 + r) & 7] = tile_f[((NAC + 2) * 4 + r) * 64 + lane];
+        if constexpr (DIR == 2)
+#pragma unroll
+          for (int j = 0; j < BPT; ++j) rws[j][(u - 2 + r) & 7] = tile_f[((NAC + 3 + j) * 4 + r) * 64 + lane];
+      }
+      if (t - 1 <= 0 || t + 2 > n) {  // a line end: ghost values
+        // both ghosts are read unconditionally and then selected (a select
+        // between addresses would put the ring in scratch memory)
+#pragma unroll
+        for (int j = 0; j < BPT; ++j) {
+          const float gl = ghost_f[(0 * BPT + j) * 64 + lane], gr = ghost_f[(1 * BPT + j) * 64 + lane];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int pos = t - 1 + r;
+            const float x = cn[j][(u - 1 + r) & 15];
+            cn[j][(u - 1 + r) & 15] = (pos <= 0) ? gl : ((pos > n) ? gr : x);
+          }
+        }
+      }
+      fetch(t + 4 * (kGD - 1), std::integral_constant<int, k>{});
     }
+    // ---- low words of the compensated state (fetched for the final update)
+    if constexpr (COMP)
+#pragma unroll
+      for (int j = 0; j < BPT; ++j)
+        lo[j][(u + kPF - 2 * NST - 1) & 7] = bload(rl[j], vc, cpos(t + kPF - 2 * NST - 1));
     // ---- coefficients of face t-2 (first used by stage 1 in this iteration)
     {
       const int sl = (u - 2) & 7;
@@ -379,7 +494,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
         const int fs = (u - 2 * (st + 1)) & 7;  // face t-2(st+1), completed by this stage
         const float d = v - sa1[j][st];
         const float m = limiter_slope<LIM>(sd[j][st], d);
-        const float cL = fmaf(0.5f, sm[j][st], sa2[j][st]);
+        const float cL = scl[j][st];
         const float cR = fmaf(-0.5f, m, sa1[j][st]);
         // G = F - P with the KT flux F = max(u,0) cL + min(u,0) cR (a = |u|: upwind)
         const float Pd = kf[fs] * sd[j][st];
@@ -396,10 +511,9 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
           Plast[j] = Pd;
         }
         const float Gp = sG[j][st];
-        sa2[j][st] = sa1[j][st];
+        scl[j][st] = fmaf(0.5f, m, sa1[j][st]);
         sa1[j][st] = v;
         sd[j][st] = d;
-        sm[j][st] = m;
         sG[j][st] = G;
         if (st < NST - 1) {
           // next stage input (position t-2st-3); the factor (0.5 dt/dxb for
@@ -422,7 +536,8 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
           bstore(rc[j], vo, cpos(q), hv);
           bstore(rl[j], vo, cpos(q), lv);
         } else {
-          bstore(rc[j], vo, cpos(q), fmaf(dtq, gq - gq1, cn[j][cl]));
+          (void)vo;
+          out_f[(j * 4 + (u & 3)) * 64 + lane] = fmaf(dtq, gq - gq1, cn[j][cl]);
         }
         if constexpr (FOUT) {  // face mass flux per unit area: faces q (1..n) and n+1
           const uint32_t fo = fpos(q), fo1 = fpos(q + 1);
@@ -431,6 +546,28 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
           bstore(rf[j], vf, fo, fmaf(a.fscale, gq, add ? bload(rf[j], vv, fo) : 0.0f));
           const uint32_t ve = (q == n) ? vf : kOOB;
           bstore(rf[j], ve, fo1, fmaf(a.fscale, gq1, add ? bload(rf[j], vv, fo1) : 0.0f));
+        }
+      }
+    }
+    // ---- group end: the 4 output rows of each bin, cells t-2NST-4 .. t-2NST-1
+    // (rows before cell 1 dropped)
+    if constexpr (!COMP && (u & 3) == 3) {
+      const int q0 = t - 3 - 2 * NST - 1;  // first output row of the group
+      const uint32_t os = xcl + (uint32_t)((q0 + 1) * (int)cs4);
+      const bool rok = (q0 + lg >= 1);
+#pragma unroll
+      for (int j = 0; j < BPT; ++j) {
+        const float4 v = ((const float4 *)out_f)[j * 64 + lane];
+        if (sfull[j] || !spart[j]) {
+          const uint32_t vo = (sfull[j] && rok) ? os : kOOB;
+          __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) uint32_t, v),
+                                                 rc[j], (int)vo, 0, 0);
+        } else {
+          const int nv = nx - col4 + 1;  // 1..3 valid columns
+          const float ve[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+          for (int e = 0; e < 3; ++e)
+            if (e < nv && rok) bstore(rc[j], os + 4 * e, 0, ve[e]);
         }
       }
     }
@@ -444,11 +581,11 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       for (int j = 0; j < BPT; ++j) {
         const float rf = fmaf(2.0f, Plast[j], Glast[j]);
         const float gq = gh[j][(u - 2 * NST - 1) & 7], gq1 = gh[j][(u - 2 * NST) & 7];
-        if (t == 2 * NST + 1) RFb[j] = rf;
-        if (t == 2 * NST + 2) tl1[j] = dtq * gq;
+        if (t == 2 * NST + 1) cap_f[(0 * BPT + j) * 64 + lane] = rf;
+        if (t == 2 * NST + 2) cap_f[(1 * BPT + j) * 64 + lane] = dtq * gq;
         if (t == tend) {
-          RFe[j] = rf;
-          trn[j] = dtq * gq1;
+          cap_f[(2 * BPT + j) * 64 + lane] = rf;
+          cap_f[(3 * BPT + j) * 64 + lane] = dtq * gq1;
         }
       }
     }
@@ -458,7 +595,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   };
 
   for (int t0 = tstart; t0 <= tend; t0 += 16)
-    unroll_range<0, 16>([&](auto U) { step(t0 + decltype(U)::value, decltype(U)::value); });
+    unroll_range<0, 16>([&](auto U) { step(t0 + decltype(U)::value, U); });
 
   double rb = 0.0, re = 0.0, lb = 0.0, le = 0.0;
   if (lane_ok) {
@@ -466,13 +603,15 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
       if (!bok[j]) continue;
-      rb += (double)RFb[j] * area * dt;
-      re += (double)RFe[j] * area * dt;
-      lb -= (double)tl1[j] * a.dxb64[1] * area;
-      le += (double)trn[j] * a.dxb64[n] * area;
+      const float RFb = cap_f[(0 * BPT + j) * 64 + lane], tl1 = cap_f[(1 * BPT + j) * 64 + lane];
+      const float RFe = cap_f[(2 * BPT + j) * 64 + lane], trn = cap_f[(3 * BPT + j) * 64 + lane];
+      rb += (double)RFb * area * dt;
+      re += (double)RFe * area * dt;
+      lb -= (double)tl1 * a.dxb64[1] * area;
+      le += (double)trn * a.dxb64[n] * area;
       if constexpr (DIR == 2) {
         const size_t ai = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(t2 + 1) + (size_t)(ny + 4) * bins[j]);
-        a.acum[ai] = a.acum[ai] + fabs((double)RFb[j]) * dt;
+        a.acum[ai] = a.acum[ai] + fabs((double)RFb) * dt;
       }
     }
   }

@@ -20,12 +20,12 @@ hipError_t launch_sweep_y(int bpt, int limiter, bool comp, int nst, const SweepA
   (void)hipGetLastError();
   if (a.fout) {  // face-flux output: minmod, FP32 state, RK4, one bin per thread only
     if (limiter != 1 || comp || nst != 4 || bpt != 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_stream<1, 1, false, 4, 1, true>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((k_stream<1, 1, false, 4, 1, true>), grid, block, (stream_lds_bytes<1, 1>(block.x)), st, a);
     return hipGetLastError();
   }
 #define F3D_L(L, CP, N, B)                                                             \
   if (limiter == L && comp == CP && nst == N && bpt == B) {                            \
-    hipLaunchKernelGGL((k_stream<1, L, CP, N, B>), grid, block, 0, st, a);             \
+    hipLaunchKernelGGL((k_stream<1, L, CP, N, B>), grid, block, (stream_lds_bytes<1, B>(block.x)), st, a);             \
     return hipGetLastError();                                                          \
   }
   F3D_STREAM_CASES(F3D_L)
@@ -38,7 +38,7 @@ hipError_t launch_sweep_y_batch(int bpt, int limiter, bool comp, int nst, const 
   (void)hipGetLastError();
 #define F3D_L(L, CP, N, B)                                                             \
   if (limiter == L && comp == CP && nst == N && bpt == B) {                            \
-    hipLaunchKernelGGL((k_stream_batch<1, L, CP, N, B>), dim3(nblocks), block, 0, st, b); \
+    hipLaunchKernelGGL((k_stream_batch<1, L, CP, N, B>), dim3(nblocks), block, (stream_lds_bytes<1, B>(block.x)), st, b); \
     return hipGetLastError();                                                          \
   }
   F3D_STREAM_CASES(F3D_L)

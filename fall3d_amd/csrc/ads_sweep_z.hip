@@ -20,7 +20,7 @@ hipError_t launch_sweep_z(int bpt, int limiter, bool comp, int nst, const SweepA
   (void)hipGetLastError();
 #define F3D_L(L, CP, N, B)                                                             \
   if (limiter == L && comp == CP && nst == N && bpt == B) {                            \
-    hipLaunchKernelGGL((k_stream<2, L, CP, N, B>), grid, block, 0, st, a);             \
+    hipLaunchKernelGGL((k_stream<2, L, CP, N, B>), grid, block, (stream_lds_bytes<2, B>(block.x)), st, a);             \
     return hipGetLastError();                                                          \
   }
   F3D_STREAM_CASES(F3D_L)
@@ -33,7 +33,7 @@ hipError_t launch_sweep_z_batch(int bpt, int limiter, bool comp, int nst, const 
   (void)hipGetLastError();
 #define F3D_L(L, CP, N, B)                                                             \
   if (limiter == L && comp == CP && nst == N && bpt == B) {                            \
-    hipLaunchKernelGGL((k_stream_batch<2, L, CP, N, B>), dim3(nblocks), block, 0, st, b); \
+    hipLaunchKernelGGL((k_stream_batch<2, L, CP, N, B>), dim3(nblocks), block, (stream_lds_bytes<2, B>(block.x)), st, b); \
     return hipGetLastError();                                                          \
   }
   F3D_STREAM_CASES(F3D_L)
