@@ -496,10 +496,12 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
   c->own_stream = true;
   c->ncell_c = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * nbins;
   c->nacum = (size_t)(nx + 4) * (ny + 4) * nbins;
-  HIPCHK(c, dalloc(&c->chi, c->ncell_c));
+  HIPCHK(c, dalloc(&c->chi_base, c->ncell_c + 2));
+  c->chi = c->chi_base + 2;
   HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
   if (c->comp) {
-    HIPCHK(c, dalloc(&c->clo, c->ncell_c));
+    HIPCHK(c, dalloc(&c->clo_base, c->ncell_c + 2));
+    c->clo = c->clo_base + 2;
     HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_c * sizeof(float), c->stream));
   }
   HIPCHK(c, dalloc(&c->acum, c->nacum));
@@ -548,7 +550,7 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
   for (auto e : c->evpool) (void)hipEventDestroy(e);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   learned_release(c);
-  void *ptrs[] = {c->chi, c->clo, c->acum, c->u1, c->u2, c->v1, c->v2, c->w1, c->w2, c->k1, c->k2, c->k3, c->vs,
+  void *ptrs[] = {c->chi_base, c->clo_base, c->acum, c->u1, c->u2, c->v1, c->v2, c->w1, c->w2, c->k1, c->k2, c->k3, c->vs,
                   c->src_idx, c->src_rate, c->src_meas, c->src_part, c->acc, c->part[1], c->part[2],
                   c->part[3], c->red_part, c->dstage};  // part[0] aliases src_part
   for (void *p : ptrs) dfree(p);

@@ -47,7 +47,11 @@ struct fall3d_ads_ctx {
   std::string err = "no error";
 
   // state
+  // c (and its low words) start 2 floats into their allocations: with the 2
+  // ghost columns, cell 1 of a row then sits on a 16-byte boundary whenever
+  // nx is a multiple of 4 (the sweeps read and write rows in 16-byte pieces)
   float *chi = nullptr, *clo = nullptr;
+  float *chi_base = nullptr, *clo_base = nullptr;
   double *acum = nullptr;
   size_t ncell_c = 0;  // (nx+4)(ny+4)(nz+4)nb
   size_t nacum = 0;    // (nx+4)(ny+4)nb
