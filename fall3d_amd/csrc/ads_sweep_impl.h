@@ -4,21 +4,21 @@
 // axis and every bin, one RK4 (or Euler) step of the 1-D KT scheme with the
 // free-flow ghosts frozen for the whole sweep. Two kernels do that here:
 //
-//  k_stream  (y and z sweeps): one thread per line; lanes of a wave hold 64
-//    consecutive x-columns, so every load/store is a coalesced 256-byte row.
-//    The thread walks its line once; the four RK stages form a software
-//    pipeline (stage s lags stage s-1 by two cells, the width of the KT
-//    stencil), so each cell is read once, written once, and no LDS or
-//    barrier is needed. Small register rings hold the last cells, face winds
-//    and face fluxes; inputs are prefetched kPF cells ahead. The waves of a
-//    workgroup are bins of the same columns, so the wind/diffusion rows they
-//    all read are shared through L1.
+//  k_stream  (y and z sweeps): one thread per line (one or two bins);
+//    lanes of a wave hold 64 consecutive x-columns. The thread walks its
+//    line once; the four RK stages form a software pipeline (stage s lags
+//    stage s-1 by two cells, the width of the KT stencil), so each cell is
+//    read once and written once. Register rings hold the last cells, face
+//    winds and face fluxes. Rows arrive in groups of 4 (one 16-byte-per-lane
+//    load per array, transposed through the wave's own LDS tile, two groups
+//    in flight) and leave the same way.
 //
 //  k_sweep_x (x sweep): lines are contiguous in memory, so one wave owns a
 //    line (or several short ones); each lane keeps C consecutive cells, the
 //    stage values and the face fluxes in VGPRs and trades two edge cells
-//    with its neighbours per stage through cross-lane permutes (no LDS, no
-//    barrier). The next bin's cells are prefetched while the current bin is
+//    with its neighbours per stage through cross-lane permutes (no LDS
+//    tile, no barrier). A lane reads and writes its cells in 16- and 8-byte
+//    pieces. The next bin's cells are prefetched while the current bin is
 //    integrated; wind and diffusion are read once per line for all bins.
 //
 // Numerics per face f (between cells f-1 and f), stage input y:
