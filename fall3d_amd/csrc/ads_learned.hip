@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -247,35 +248,66 @@ __device__ __forceinline__ size_t face_index(const FaceArgs &a, int f, int tr, i
 // ---------------------------------------------------------------------------
 // inference: dPhi of every interior face of one axis (others stay zero)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedWeights *__restrict__ w,
-                                                       float *__restrict__ dphi) {
-  const int l = threadIdx.x & 63, col = l & 15;
+// Activity pass: each wave takes 64 consecutive tiles, four at a time (one
+// per 16-lane group). A tile whose faces have both cells at or below the
+// threshold gets zero corrections here; the others are appended to the list
+// (one atomic per wave; the order of the list does not matter, every face is
+// computed from its own stencil).
+__global__ void __launch_bounds__(256) k_active_tiles(FaceArgs a, float *__restrict__ dphi, int *__restrict__ list,
+                                                      unsigned *__restrict__ count) {
+  const int l = threadIdx.x & 63, col = l & 15, g = l >> 4;
   int nt;
   long rows;
   face_space(a.axis, a.nx, a.ny, a.nz, a.nb, nt, rows);
   const long ntile = (long)nt * rows;
+  const long chunk = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long t0 = chunk * 64;
+  if (t0 >= ntile) return;
+  const float thr = fmaxf(a.smin_dev ? *a.smin_dev : a.smin, kSmallest);
+  uint64_t mask = 0;
+#pragma unroll 4
+  for (int it = 0; it < 16; ++it) {
+    const long t = t0 + 4 * it + g;
+    int f = 0, tr = 0, k = 0, b = 0;
+    const bool ok = t < ntile && tile_face(a, t, col, f, tr, k, b);
+    bool live = false;
+    if (ok) {
+      const int i0 = (a.axis == 0) ? f - 1 : tr, j0 = (a.axis == 0) ? tr : f - 1;
+      const int i1 = (a.axis == 0) ? f : tr, j1 = (a.axis == 0) ? tr : f;
+      live = fmaxf(fabsf(a.c[cix(a, i0, j0, k, b)]), fabsf(a.c[cix(a, i1, j1, k, b)])) > thr;
+    }
+    const uint64_t bal = __ballot(live);
+    for (int q = 0; q < 4; ++q)
+      if ((bal >> (16 * q)) & 0xffffu) mask |= (uint64_t)1 << (4 * it + q);
+    if (ok && !((bal >> (16 * g)) & 0xffffu)) dphi[face_index(a, f, tr, k, b)] = 0.0f;
+  }
+  const int na = __popcll(mask);
+  if (na == 0) return;
+  unsigned base = 0;
+  if (l == 0) base = atomicAdd(count, (unsigned)na);
+  base = __shfl(base, 0, 64);
+  if ((mask >> l) & 1) {
+    const int rank = __popcll(mask & (((uint64_t)1 << l) - 1));
+    list[base + rank] = (int)(t0 + l);
+  }
+}
+
+// inference: dPhi of the faces of the active tiles (the activity pass has
+// zeroed the others)
+__global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedWeights *__restrict__ w,
+                                                       float *__restrict__ dphi, const int *__restrict__ list,
+                                                       const unsigned *__restrict__ count) {
+  const int l = threadIdx.x & 63, col = l & 15;
+  const long na = *count;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long nwave = (long)gridDim.x * (blockDim.x >> 6);
   bf16x8 w1t[4], w2t[4][2];
   load_weights(w, w1t, w2t);
   const float smin = a.smin_dev ? *a.smin_dev : a.smin;
-  for (long t = wave; t < ntile; t += nwave) {
+  for (long q = wave; q < na; q += nwave) {
+    const long t = list[q];
     int f, tr, k, b;
     const bool ok = tile_face(a, t, col, f, tr, k, b);
-    // cheap activity test before the stencil gather: a tile whose faces
-    // have both cells at or below the threshold gets no correction
-    {
-      bool live = false;
-      if (ok) {
-        const int i0 = (a.axis == 0) ? f - 1 : tr, j0 = (a.axis == 0) ? tr : f - 1;
-        const int i1 = (a.axis == 0) ? f : tr, j1 = (a.axis == 0) ? tr : f;
-        live = fmaxf(fabsf(a.c[cix(a, i0, j0, k, b)]), fabsf(a.c[cix(a, i1, j1, k, b)])) > fmaxf(smin, kSmallest);
-      }
-      if (!__ballot(live)) {
-        if (ok && l < 16) dphi[face_index(a, f, tr, k, b)] = 0.0f;
-        continue;
-      }
-    }
     FaceGeo geo;
     if (ok) geo = face_geo(a, f, tr, k, b);
     float S = 0.0f;
@@ -301,6 +333,10 @@ __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedW
     }
     if (ok && l < 16) dphi[face_index(a, f, tr, k, b)] = out;
   }
+}
+
+__global__ void k_zero_counts(unsigned *__restrict__ count, int n) {
+  if (threadIdx.x < n) count[threadIdx.x] = 0u;
 }
 
 // correction of one face, limited to lim |plain face flux| (lim <= 0: off)
@@ -561,13 +597,18 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "learned threshold: %s", hipGetErrorString(e));
   }
+  // a fill kernel, not a memset: the step may be captured in a graph
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, c->stream, c->lcount, 2);
   for (int axis = 0; axis < 2; ++axis) {
     FaceArgs a = face_args(c, axis, dt, stime, rel ? 0.0 : c->learned_smin);
     if (rel) a.smin_dev = c->lthr;
     a.gain = (float)c->learned_gain;
     const long nt = ntiles(c, axis);
+    const long chunks = (nt + 63) / 64;  // 64 tiles per wave, 4 waves per workgroup
+    hipLaunchKernelGGL(k_active_tiles, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, c->stream, a, c->dphi[axis],
+                       c->lact[axis], c->lcount + axis);
     hipLaunchKernelGGL(k_correct_faces, dim3(grid_for(nt)), dim3(256), 0, c->stream, a, c->learned->packed,
-                       c->dphi[axis]);
+                       c->dphi[axis], c->lact[axis], c->lcount + axis);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "face corrections: %s", hipGetErrorString(e));
   }
@@ -594,6 +635,12 @@ void learned_release(fall3d_ads_ctx *c) {
   }
   dfree(c->lthr);
   c->lthr = nullptr;
+  for (int a = 0; a < 2; ++a) {
+    dfree(c->lact[a]);
+    c->lact[a] = nullptr;
+  }
+  dfree(c->lcount);
+  c->lcount = nullptr;
   c->learned = nullptr;
 }
 
@@ -820,8 +867,12 @@ int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, doub
     if (c->comp) return fail(c, FALL3D_ADS_EINVAL, "the learned correction runs on the FP32 state only");
     if (flux_limit > 0.0 && (c->limiter != 1 || c->tm != 2))
       return fail(c, FALL3D_ADS_EINVAL, "the flux-relative limit needs minmod and RK4");
+    if (ntiles(c, 0) > INT_MAX || ntiles(c, 1) > INT_MAX)
+      return fail(c, FALL3D_ADS_EINVAL, "domain too large for the learned correction's tile list");
+    if (!c->lcount) HIPCHK(c, dalloc(&c->lcount, 2));
     for (int a = 0; a < 2; ++a) {
       if (!c->dphi[a]) HIPCHK(c, dalloc(&c->dphi[a], nface(c, a)));
+      if (!c->lact[a]) HIPCHK(c, dalloc(&c->lact[a], (size_t)ntiles(c, a)));
       HIPCHK(c, hipMemsetAsync(c->dphi[a], 0, sizeof(float) * nface(c, a), c->stream));
       if (flux_limit > 0.0 && !c->fout[a]) {
         HIPCHK(c, dalloc(&c->fout[a], nface(c, a)));
