@@ -116,6 +116,7 @@ struct fall3d_ads_ctx {
   struct fall3d_mlp *learned = nullptr;  // flux-correction network (borrowed) or nullptr: plain KT
   float *dphi[2] = {nullptr, nullptr};   // its face corrections of the current step
   int *lact[2] = {nullptr, nullptr};     // active 16-face tiles of each axis (compacted list)
+  unsigned char *lflag[2] = {nullptr, nullptr};  // per tile: 1 = active this step
   unsigned *lcount = nullptr;            // their counts (2)
   double learned_smin = 0.0;
   double learned_gain = 1.0;

@@ -205,13 +205,14 @@ __device__ __forceinline__ bf16x8 face_features(const FaceArgs &a, const FaceGeo
   return xb;
 }
 
-// number of interior faces along the tile direction (x index) and rows
+// tiles of 16 faces along x per row (a multiple of 4: rows start on whole
+// 64-face groups, the tiles past the last face are empty) and rows
 __device__ __host__ __forceinline__ void face_space(int axis, int nx, int ny, int nz, int nb, int &nt, long &rows) {
   if (axis == 0) {  // faces f = 2..nx along x; rows (j, k, b)
-    nt = (nx - 1 + 15) / 16;
+    nt = (nx - 1 + 63) / 64 * 4;
     rows = (long)ny * nz * nb;
   } else {  // cells i = 1..nx along x; rows (f = 2..ny, k, b)
-    nt = (nx + 15) / 16;
+    nt = (nx + 63) / 64 * 4;
     rows = (long)(ny - 1) * nz * nb;
   }
 }
@@ -221,19 +222,21 @@ __device__ __forceinline__ bool tile_face(const FaceArgs &a, long t, int col, in
   int nt;
   long rows;
   face_space(a.axis, a.nx, a.ny, a.nz, a.nb, nt, rows);
-  const long row = t / nt;
-  const int x = (int)(t % nt) * 16 + col;
+  // 32-bit index arithmetic: the host keeps the tile count below 2^31
+  const unsigned tu = (unsigned)t, ntu = (unsigned)nt;
+  const unsigned row = tu / ntu;
+  const int x = (int)(tu - row * ntu) * 16 + col;
+  const unsigned ry = (unsigned)((a.axis == 0) ? a.ny : a.ny - 1), kb = row / ry;
+  const unsigned kk = kb / (unsigned)a.nz;
+  k = (int)(kb - kk * (unsigned)a.nz) + 1;
+  b = (int)kk;
   if (a.axis == 0) {
     f = x + 2;
-    tr = (int)(row % a.ny) + 1;
-    k = (int)((row / a.ny) % a.nz) + 1;
-    b = (int)(row / ((long)a.ny * a.nz));
+    tr = (int)(row - kb * ry) + 1;
     return f <= a.nx;
   } else {
     tr = x + 1;
-    f = (int)(row % (a.ny - 1)) + 2;
-    k = (int)((row / (a.ny - 1)) % a.nz) + 1;
-    b = (int)(row / ((long)(a.ny - 1) * a.nz));
+    f = (int)(row - kb * ry) + 2;
     return tr <= a.nx;
   }
 }
@@ -248,39 +251,68 @@ __device__ __forceinline__ size_t face_index(const FaceArgs &a, int f, int tr, i
 // ---------------------------------------------------------------------------
 // inference: dPhi of every interior face of one axis (others stay zero)
 // ---------------------------------------------------------------------------
-// Activity pass: each wave takes 64 consecutive tiles, four at a time (one
-// per 16-lane group). A tile whose faces have both cells at or below the
-// threshold gets zero corrections here; the others are appended to the list
-// (one atomic per wave; the order of the list does not matter, every face is
-// computed from its own stencil).
-__global__ void __launch_bounds__(256) k_active_tiles(FaceArgs a, float *__restrict__ dphi, int *__restrict__ list,
-                                                      unsigned *__restrict__ count) {
-  const int l = threadIdx.x & 63, col = l & 15, g = l >> 4;
+// Activity pass: each wave takes 16 consecutive 64-face groups (4 tiles
+// each; a group never straddles rows), one face per lane. A tile is active
+// when one of its faces has a cell above the threshold; the pass writes one
+// flag per tile and appends the active tiles to the list (one atomic per
+// wave; the order of the list does not matter, every face is computed from
+// its own stencil). Faces of inactive tiles get no correction: the apply pass
+// reads their flag, not their dPhi. The host keeps the tile count below 2^31.
+__global__ void __launch_bounds__(256) k_active_tiles(FaceArgs a, unsigned char *__restrict__ flag,
+                                                      int *__restrict__ list, unsigned *__restrict__ count) {
+  const int l = threadIdx.x & 63;
   int nt;
   long rows;
   face_space(a.axis, a.nx, a.ny, a.nz, a.nb, nt, rows);
-  const long ntile = (long)nt * rows;
-  const long chunk = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const long t0 = chunk * 64;
-  if (t0 >= ntile) return;
+  const unsigned ntile = (unsigned)nt * (unsigned)rows;
+  const int ngr = nt >> 2;  // 64-face groups per row
+  const unsigned ngroups = ntile >> 2;
+  const unsigned G0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u;
+  if (G0 >= ngroups) return;
+  const unsigned t0 = 4 * G0;
   const float thr = fmaxf(a.smin_dev ? *a.smin_dev : a.smin, kSmallest);
-  uint64_t mask = 0;
-#pragma unroll 4
+  const int nfr = (a.axis == 0) ? a.nx - 1 : a.nx;  // faces per row
+  const int ry = (a.axis == 0) ? a.ny : a.ny - 1;   // rows per level
+  // row = r + ry (k + nz b), all 0-based; the first face's cells are
+  // (1, r + 1) and, along the face normal, the next one
+  const unsigned row0 = G0 / (unsigned)ngr;
+  int xg = (int)(G0 - row0 * (unsigned)ngr);
+  const unsigned kb0 = row0 / (unsigned)ry;
+  int r = (int)(row0 - kb0 * (unsigned)ry);
+  int k = (int)(kb0 % (unsigned)a.nz), b = (int)(kb0 / (unsigned)a.nz);
+  const size_t d2 = (a.axis == 0) ? 1 : (size_t)(a.nx + 4);
+  size_t cb = cix(a, 1, r + 1, k + 1, b);
+  // all 32 loads first (faces past the end read element 0), then the tests
+  float v0[16], v1[16];
+  unsigned okb = 0;
+#pragma unroll
   for (int it = 0; it < 16; ++it) {
-    const long t = t0 + 4 * it + g;
-    int f = 0, tr = 0, k = 0, b = 0;
-    const bool ok = t < ntile && tile_face(a, t, col, f, tr, k, b);
-    bool live = false;
-    if (ok) {
-      const int i0 = (a.axis == 0) ? f - 1 : tr, j0 = (a.axis == 0) ? tr : f - 1;
-      const int i1 = (a.axis == 0) ? f : tr, j1 = (a.axis == 0) ? tr : f;
-      live = fmaxf(fabsf(a.c[cix(a, i0, j0, k, b)]), fabsf(a.c[cix(a, i1, j1, k, b)])) > thr;
+    const int x = xg * 64 + l;
+    const bool ok = G0 + it < ngroups && x < nfr;
+    v0[it] = a.c[ok ? cb + x : 0];
+    v1[it] = a.c[ok ? cb + x + d2 : 0];
+    okb |= (unsigned)ok << it;
+    if (++xg == ngr) {  // next row
+      xg = 0;
+      if (++r == ry) {
+        r = 0;
+        if (++k == a.nz) {
+          k = 0;
+          ++b;
+        }
+      }
+      cb = cix(a, 1, r + 1, k + 1, b);
     }
-    const uint64_t bal = __ballot(live);
-    for (int q = 0; q < 4; ++q)
-      if ((bal >> (16 * q)) & 0xffffu) mask |= (uint64_t)1 << (4 * it + q);
-    if (ok && !((bal >> (16 * g)) & 0xffffu)) dphi[face_index(a, f, tr, k, b)] = 0.0f;
   }
+  uint64_t mask = 0;
+#pragma unroll
+  for (int it = 0; it < 16; ++it) {
+    const bool live = ((okb >> it) & 1) && fmaxf(fabsf(v0[it]), fabsf(v1[it])) > thr;
+    const uint64_t bal = __ballot(live);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mask |= (uint64_t)(((bal >> (16 * q)) & 0xffffu) != 0) << (4 * it + q);
+  }
+  if (t0 + l < ntile) flag[t0 + l] = (unsigned char)((mask >> l) & 1);
   const int na = __popcll(mask);
   if (na == 0) return;
   unsigned base = 0;
@@ -292,8 +324,7 @@ __global__ void __launch_bounds__(256) k_active_tiles(FaceArgs a, float *__restr
   }
 }
 
-// inference: dPhi of the faces of the active tiles (the activity pass has
-// zeroed the others)
+// inference: dPhi of every face of the active tiles
 __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedWeights *__restrict__ w,
                                                        float *__restrict__ dphi, const int *__restrict__ list,
                                                        const unsigned *__restrict__ count) {
@@ -339,34 +370,99 @@ __global__ void k_zero_counts(unsigned *__restrict__ count, int n) {
   if (threadIdx.x < n) count[threadIdx.x] = 0u;
 }
 
-// correction of one face, limited to lim |plain face flux| (lim <= 0: off)
+// correction of one face, limited to lim |plain face flux| (lim <= 0: off);
+// zero for a face outside every active tile
 __device__ __forceinline__ float limited(const float *__restrict__ d, const float *__restrict__ phi, size_t i,
-                                         float lim) {
+                                         float lim, bool on) {
+  if (!on) return 0.0f;
   const float v = d[i];
   if (lim <= 0.0f) return v;
   const float m = lim * fabsf(phi[i]);
   return fminf(fmaxf(v, -m), m);
 }
-
-// c -= (dX(i+1) - dX(i)) / dxb_x(i) + (dY(j+1) - dY(j)) / dxb_y(j); both
-// cells of a face see the same limited value, so the update stays
-// conservative
-__global__ void k_apply_corrections(float *__restrict__ c, const float *__restrict__ dx, const float *__restrict__ dy,
-                                    const float *__restrict__ px, const float *__restrict__ py, float lim,
-                                    int nx, int ny, int nz, int nb, const float *__restrict__ idxb_x,
-                                    const float *__restrict__ idxb_y) {
-  const size_t n = (size_t)nx * ny * nz * nb;
-  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(q % nx) + 1;
-    const int j = (int)((q / nx) % ny) + 1;
-    const int k = (int)((q / ((size_t)nx * ny)) % nz) + 1;
-    const int b = (int)(q / ((size_t)nx * ny * nz));
-    const size_t fx = (size_t)i + (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * ((size_t)(k - 1) + (size_t)nz * b));
-    const size_t fy = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * ((size_t)(k - 1) + (size_t)nz * b));
-    const float d = (limited(dx, px, fx + 1, lim) - limited(dx, px, fx, lim)) * idxb_x[i] +
-                    (limited(dy, py, fy + nx, lim) - limited(dy, py, fy, lim)) * idxb_y[j];
-    const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
-    c[ci] -= d;
+// Each wave takes a contiguous range of 64-cell segments of x rows (j, k, b),
+// nseg segments per row, nsegs of them in all (the host keeps that below
+// 2^31), eight segments at a time. The tile flags that a segment needs are
+// wave-uniform: the 4 x tiles of its faces 64 sx + 2 .. 64 sx + 65 (one
+// 32-bit word, tile rows being whole words), the x tile before them (face
+// 64 sx + 1), and the 4 y tiles of the face rows j and j + 1. A segment whose
+// flags are all zero has no correction and is skipped; in the others a cell
+// none of whose four faces lies in an active tile is left untouched.
+__global__ void __launch_bounds__(256) k_apply_corrections(
+    float *__restrict__ c, const float *__restrict__ dx, const float *__restrict__ dy, const float *__restrict__ px,
+    const float *__restrict__ py, const unsigned char *__restrict__ flx, const unsigned char *__restrict__ fly,
+    float lim, int nx, int ny, int nz, int nb, int nseg, unsigned per_wave, const float *__restrict__ idxb_x,
+    const float *__restrict__ idxb_y) {
+  int ntx, nty;  // tiles per face row of each axis (multiples of 4)
+  long rows_unused;
+  face_space(0, nx, ny, nz, nb, ntx, rows_unused);
+  face_space(1, nx, ny, nz, nb, nty, rows_unused);
+  const int nwx = ntx >> 2, nwy = nty >> 2;  // flag words per face row
+  const unsigned *__restrict__ wfx = reinterpret_cast<const unsigned *>(flx);
+  const unsigned *__restrict__ wfy = reinterpret_cast<const unsigned *>(fly);
+  const unsigned nsegs = (unsigned)nseg * (unsigned)ny * (unsigned)nz * (unsigned)nb;
+  const unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const unsigned s0 = wave * per_wave;
+  if (s0 >= nsegs) return;
+  const unsigned s1 = min(nsegs, s0 + per_wave);
+  const int l = threadIdx.x & 63;
+  // segment position: sx along the row, row = (j - 1) + ny kb, kb = (k - 1) + nz b
+  const unsigned row0 = s0 / (unsigned)nseg;
+  int sx = (int)(s0 - row0 * (unsigned)nseg);
+  const unsigned kb0 = row0 / (unsigned)ny;
+  int j = (int)(row0 - kb0 * (unsigned)ny) + 1;
+  int kb = (int)kb0;
+  constexpr int U = 8;
+  for (unsigned sg = s0; sg < s1; sg += U) {
+    unsigned wx[U], wp[U], wy0[U], wy1[U];
+    int sj[U], skb[U], ssx[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {  // flag loads of eight segments (invalid ones read word 0)
+      const bool v = sg + u < s1;
+      const size_t rx = (size_t)(j - 1) + (size_t)ny * kb;  // x-face row of these cells
+      const bool vx = v && sx < nwx, vp = v && sx > 0, vy0 = v && j >= 2, vy1 = v && j + 1 <= ny;
+      wx[u] = wfx[vx ? rx * nwx + sx : 0];
+      wp[u] = flx[vp ? rx * ntx + 4 * sx - 1 : 0];
+      wy0[u] = wfy[vy0 ? ((size_t)(j - 2) + (size_t)(ny - 1) * kb) * nwy + sx : 0];
+      wy1[u] = wfy[vy1 ? ((size_t)(j - 1) + (size_t)(ny - 1) * kb) * nwy + sx : 0];
+      wx[u] = vx ? wx[u] : 0u;
+      wp[u] = vp ? wp[u] : 0u;
+      wy0[u] = vy0 ? wy0[u] : 0u;
+      wy1[u] = vy1 ? wy1[u] : 0u;
+      sj[u] = j;
+      skb[u] = kb;
+      ssx[u] = sx;
+      if (++sx == nseg) {  // next row
+        sx = 0;
+        if (++j > ny) {
+          j = 1;
+          ++kb;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if ((wx[u] | wp[u] | wy0[u] | wy1[u]) == 0u) continue;  // uniform
+      const int i = ssx[u] * 64 + l + 1, jj = sj[u];
+      if (i > nx) continue;
+      // tile of face i: the word's tile ((l - 1) >> 4) (lane 0: the tile
+      // before); of face i + 1: tile l >> 4; of the y faces: tile l >> 4
+      const uint64_t X = (uint64_t)wp[u] | ((uint64_t)wx[u] << 8);
+      const int tl0 = (l == 0) ? 0 : ((l - 1) >> 4) + 1, tl1 = (l >> 4) + 1;
+      const bool ox0 = i >= 2 && ((X >> (8 * tl0)) & 1);
+      const bool ox1 = i + 1 <= nx && ((X >> (8 * tl1)) & 1);
+      const bool oy0 = (wy0[u] >> (8 * (l >> 4))) & 1;
+      const bool oy1 = (wy1[u] >> (8 * (l >> 4))) & 1;
+      if (!(ox0 || ox1 || oy0 || oy1)) continue;
+      const size_t kbs = (size_t)skb[u];
+      const int k = skb[u] % nz + 1, b = skb[u] / nz;
+      const size_t fx = (size_t)i + (size_t)(nx + 3) * ((size_t)(jj - 1) + (size_t)ny * kbs);
+      const size_t fy = (size_t)(i - 1) + (size_t)nx * ((size_t)jj + (size_t)(ny + 3) * kbs);
+      const float d = (limited(dx, px, fx + 1, lim, ox1) - limited(dx, px, fx, lim, ox0)) * idxb_x[i] +
+                      (limited(dy, py, fy + nx, lim, oy1) - limited(dy, py, fy, lim, oy0)) * idxb_y[jj];
+      const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(jj + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
+      c[ci] -= d;
+    }
   }
 }
 
@@ -604,8 +700,8 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
     if (rel) a.smin_dev = c->lthr;
     a.gain = (float)c->learned_gain;
     const long nt = ntiles(c, axis);
-    const long chunks = (nt + 63) / 64;  // 64 tiles per wave, 4 waves per workgroup
-    hipLaunchKernelGGL(k_active_tiles, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, c->stream, a, c->dphi[axis],
+    const long chunks = (nt + 63) / 64;  // 64 tiles (16 groups) per wave, 4 waves per workgroup
+    hipLaunchKernelGGL(k_active_tiles, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, c->stream, a, c->lflag[axis],
                        c->lact[axis], c->lcount + axis);
     hipLaunchKernelGGL(k_correct_faces, dim3(grid_for(nt)), dim3(256), 0, c->stream, a, c->learned->packed,
                        c->dphi[axis], c->lact[axis], c->lcount + axis);
@@ -616,12 +712,15 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
 }
 
 int learned_post_step(fall3d_ads_ctx *c) {
-  const size_t n = (size_t)c->nx * c->ny * c->nz * c->nb;
-  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  const int nseg = (c->nx + 63) / 64;
+  const long nsegs = (long)c->ny * c->nz * c->nb * nseg;
+  const long per_wave = std::max<long>(8, (nsegs + 65535) / 65536);  // at most 65536 waves
+  const unsigned blocks = (unsigned)((nsegs + 4 * per_wave - 1) / (4 * per_wave));
   const float lim = (float)c->learned_flux_limit;
   hipLaunchKernelGGL(k_apply_corrections, dim3(blocks), dim3(256), 0, c->stream, c->chi, c->dphi[0], c->dphi[1],
-                     lim > 0.0f ? c->fout[0] : c->dphi[0], lim > 0.0f ? c->fout[1] : c->dphi[1], lim, c->nx, c->ny,
-                     c->nz, c->nb, c->idxb[0], c->idxb[1]);
+                     lim > 0.0f ? c->fout[0] : c->dphi[0], lim > 0.0f ? c->fout[1] : c->dphi[1], c->lflag[0],
+                     c->lflag[1], lim, c->nx, c->ny, c->nz, c->nb, nseg, (unsigned)per_wave, c->idxb[0],
+                     c->idxb[1]);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "apply corrections: %s", hipGetErrorString(e));
   return FALL3D_ADS_OK;
@@ -637,7 +736,9 @@ void learned_release(fall3d_ads_ctx *c) {
   c->lthr = nullptr;
   for (int a = 0; a < 2; ++a) {
     dfree(c->lact[a]);
+    dfree(c->lflag[a]);
     c->lact[a] = nullptr;
+    c->lflag[a] = nullptr;
   }
   dfree(c->lcount);
   c->lcount = nullptr;
@@ -867,12 +968,13 @@ int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, doub
     if (c->comp) return fail(c, FALL3D_ADS_EINVAL, "the learned correction runs on the FP32 state only");
     if (flux_limit > 0.0 && (c->limiter != 1 || c->tm != 2))
       return fail(c, FALL3D_ADS_EINVAL, "the flux-relative limit needs minmod and RK4");
-    if (ntiles(c, 0) > INT_MAX || ntiles(c, 1) > INT_MAX)
+    if (ntiles(c, 0) > INT_MAX || ntiles(c, 1) > INT_MAX || (long)c->ny * c->nz * c->nb * ((c->nx + 63) / 64) > INT_MAX)
       return fail(c, FALL3D_ADS_EINVAL, "domain too large for the learned correction's tile list");
     if (!c->lcount) HIPCHK(c, dalloc(&c->lcount, 2));
     for (int a = 0; a < 2; ++a) {
       if (!c->dphi[a]) HIPCHK(c, dalloc(&c->dphi[a], nface(c, a)));
       if (!c->lact[a]) HIPCHK(c, dalloc(&c->lact[a], (size_t)ntiles(c, a)));
+      if (!c->lflag[a]) HIPCHK(c, dalloc(&c->lflag[a], (size_t)ntiles(c, a)));
       HIPCHK(c, hipMemsetAsync(c->dphi[a], 0, sizeof(float) * nface(c, a), c->stream));
       if (flux_limit > 0.0 && !c->fout[a]) {
         HIPCHK(c, dalloc(&c->fout[a], nface(c, a)));
@@ -899,6 +1001,8 @@ int fall3d_ads_learned_collect(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse, fal
   if (fine->nx != 2 * coarse->nx || fine->ny != 2 * coarse->ny || fine->nz != coarse->nz || fine->nb != coarse->nb)
     return nfail(n, FALL3D_ADS_EINVAL, "fine grid must be the 2x horizontal refinement of the coarse grid");
   if (n->cap == 0) return nfail(n, FALL3D_ADS_ESTATE, "network created without training capacity");
+  if (ntiles(coarse, 0) > INT_MAX || ntiles(coarse, 1) > INT_MAX)
+    return nfail(n, FALL3D_ADS_EINVAL, "coarse domain too large for the face tiles");
   int rc;
   hipStream_t st = coarse->stream;
   if (fine->stream != st) HIPCHK(fine, hipStreamSynchronize(fine->stream));

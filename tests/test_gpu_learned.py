@@ -61,6 +61,105 @@ def test_rows_match_the_cpu_restatement():
     coarse.release()
 
 
+def _corrections_cpu(prob, c0, dt, net, smin, gain):
+    """Face corrections (dPhi per axis, device face layouts) of one learned
+    step restated on the CPU: a 16-face tile along x is active when one of
+    its faces has a cell above max(smin, 1e-30); every face of an active tile
+    with stencil maximum S > smin gets gain sgn clamp(S dxp 0.01 y), y the
+    network on the face's features, clamped to a quarter of the donor
+    cell's mass; all other faces get none."""
+    from oracle.learned_oracle import FaceFeatures
+
+    f32 = np.float32
+    nx, ny, nz, nb = prob.nx, prob.ny, prob.nz, prob.nb
+    c = np.asarray(c0, dtype=np.float32)
+    ff = FaceFeatures(prob, c, dt)
+    thr = max(f32(smin), f32(1e-30))
+    out = [np.zeros((nx + 3, ny, nz, nb), np.float32), np.zeros((nx, ny + 3, nz, nb), np.float32)]
+    faces, bits, stats = [], [], {"tiles": 0, "active": 0}
+    for axis in (0, 1):
+        nf, rows = (nx - 1, [(tr, k, b) for b in range(nb) for k in range(1, nz + 1) for tr in range(1, ny + 1)]) \
+            if axis == 0 else (nx, [(f, k, b) for b in range(nb) for k in range(1, nz + 1) for f in range(2, ny + 1)])
+        for (r, k, b) in rows:
+            for t0 in range(0, nf, 16):
+                xs = range(t0, min(t0 + 16, nf))
+                if axis == 0:  # face f = x + 2 between cells f - 1 and f of row j = r
+                    fl = [(x + 2, r, (x + 1, r), (x + 2, r)) for x in xs]
+                else:  # cell i = x + 1, face f = r between rows f - 1 and f
+                    fl = [(r, x + 1, (x + 1, r - 1), (x + 1, r)) for x in xs]
+                stats["tiles"] += 1
+                if not any(max(abs(c[i0 + 1, j0 + 1, k + 1, b]), abs(c[i1 + 1, j1 + 1, k + 1, b])) > thr
+                           for (_, _, (i0, j0), (i1, j1)) in fl):
+                    continue
+                stats["active"] += 1
+                for (f, tr, L, R) in fl:
+                    fb, S, den = ff.face(axis, b, f, tr, k)
+                    if S > f32(smin):
+                        faces.append((axis, f, tr, k, b, L, R, S, den))
+                        bits.append(fb)
+    y = net.forward(np.array(bits, dtype=np.uint16)) if bits else np.zeros(0, np.float32)
+    for (axis, f, tr, k, b, L, R, S, den), yv in zip(faces, y):
+        sgn = f32(1.0) if den > 0 else f32(-1.0)
+        U, D = (L, R) if sgn > 0 else (R, L)
+        dxp = f32(f32(0.5) / ff.hdxp[axis][f])
+        F = f32(f32(f32(S * dxp) * f32(0.01)) * yv)
+        du = f32(f32(1.0) / ff.idxb[axis][U[axis]])
+        dd = f32(f32(1.0) / ff.idxb[axis][D[axis]])
+        F = min(F, f32(f32(f32(0.25) * max(c[U[0] + 1, U[1] + 1, k + 1, b], f32(0.0))) * du))
+        F = max(F, f32(f32(f32(-0.25) * max(c[D[0] + 1, D[1] + 1, k + 1, b], f32(0.0))) * dd))
+        v = f32(f32(f32(gain) * sgn) * F)
+        if axis == 0:
+            out[0][f, tr - 1, k - 1, b] = v
+        else:
+            out[1][tr - 1, f, k - 1, b] = v
+    return out, stats
+
+
+def test_learned_step_applies_the_network_corrections():
+    """One learned step equals the plain step minus the divergence of the
+    face corrections restated on the CPU (features from the oracle, y from
+    the network's own forward pass, which is pinned against the MLP oracle
+    separately). A ragged grid (tiles of 16 faces with partial last tiles),
+    a threshold that leaves some tiles inactive, and an output layer scaled
+    so that part of the faces reach the positivity clamp."""
+    from fall3d_amd.problem import config1_wind_problem
+
+    p = config1_wind_problem(37, 29, 6, 2, seed=5, dx=5000.0, dz=1000.0, nsources=1, puff=True)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    g.set_problem(p)
+    dt = g.time_step(0.0, 2, 0.9)
+    g.run(dt, 4, 0)
+    c0, a0 = g.get_state()
+    params = ads.Network.initial_params(11)
+    params[-65:] *= 300.0  # w3, b3: outputs large enough to meet the clamp
+    net = ads.Network(params=params)
+    smin, gain = 1e-3 * float(np.abs(c0).max()), 0.5
+    res = []
+    for learned in (False, True):
+        g.set_state(c0, a0)
+        g.set_learned(net if learned else None, smin=smin, gain=gain)
+        g.step(dt, 0)
+        res.append(g.get_state()[0])
+    g.release()
+    dphi, stats = _corrections_cpu(p, c0, dt, net, smin, gain)
+    print(f"tiles {stats}", flush=True)
+    assert 0 < stats["active"] < stats["tiles"]
+    nx, ny = p.nx, p.ny
+    ix = [np.asarray(1.0 / np.asarray(p.grid[k], dtype=np.float64), dtype=np.float32) for k in ("dX1_b", "dX2_b")]
+    dx, dy = dphi
+    d = (f32d(dx[2:nx + 2] - dx[1:nx + 1]) * ix[0][1:nx + 1, None, None, None]
+         + f32d(dy[:, 2:ny + 2] - dy[:, 1:ny + 1]) * ix[1][None, 1:ny + 1, None, None]).astype(np.float32)
+    got = (res[1] - res[0])[2:nx + 2, 2:ny + 2, 2:p.nz + 2, :]
+    assert np.abs(d).max() > 0
+    err = np.abs(got + d).max()
+    print(f"max |correction| {np.abs(d).max():.3e}, max err {err:.3e}", flush=True)
+    assert err <= 1e-4 * np.abs(d).max() + 1e-6 * np.abs(res[0]).max()
+
+
+def f32d(a):
+    return np.asarray(a, dtype=np.float32)
+
+
 def test_trained_correction_beats_plain_coarse_kt():
     net = ads.Network(capacity=3_000_000)
     n = learned.collect(net, range(6), SIZE, nwarm=16, nsamples=6, gap=2, **KW)
