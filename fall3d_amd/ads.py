@@ -423,7 +423,9 @@ class ADS:
         self._chk(load_library().fall3d_ads_set_graph(self._h, 1 if on else 0), "set_graph")
 
     def set_profiling(self, on=True):
-        self._chk(load_library().fall3d_ads_set_profiling(self._h, 1 if on else 0), "set_profiling")
+        """on: False/0 off, True/1 time every step, N > 1 time one step in N of run()
+        (graph replays in between)."""
+        self._chk(load_library().fall3d_ads_set_profiling(self._h, int(on)), "set_profiling")
 
     def kernel_time(self, kernel):
         ms, n = _dbl(), ctypes.c_longlong()

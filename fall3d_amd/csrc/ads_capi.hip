@@ -883,30 +883,30 @@ int fall3d_ads_step(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
 int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int nsteps) {
   if (!c) return FALL3D_ADS_EINVAL;
   int rc;
-  int it = iiter0, left = nsteps;
   // Launch-bound inner loop: replay a captured (even, odd) step pair as one
-  // hipGraph. Per-kernel timing needs events between kernels, so profiling
-  // runs launch directly.
-  const bool graph = c->use_graph && !c->prof && nsteps >= 4;
+  // hipGraph. Per-kernel timing needs events between kernels, so timed steps
+  // launch directly: all of them when every step is timed (no graph), one in
+  // prof_every otherwise (the pairs in between are replays).
+  const int every = c->prof_every;
+  const bool sampling = every > 1;
+  const bool graph = c->use_graph && every != 1 && nsteps >= 4;
   if (graph) {
     // the dt/dxb tables are refreshed outside the graph (they may be stale
     // after fall3d_ads_set_grid even when the graph itself is reused)
     if ((rc = ensure_dt(c, dt))) return rc;
-    if (it % 2 != 0) {
-      if ((rc = fall3d_ads_step(c, dt, stime, it))) return rc;
-      ++it;
-      --left;
-    }
     if (!c->gexec || c->g_dt != dt || c->g_stime != stime || c->g_version != c->version) {
       if (c->gexec) {
         (void)hipGraphExecDestroy(c->gexec);
         c->gexec = nullptr;
       }
+      const bool prof = c->prof;
+      c->prof = false;  // no events inside the graph
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       int r0 = fall3d_ads_step(c, dt, stime, 0);
       int r1 = r0 ? r0 : fall3d_ads_step(c, dt, stime, 1);
       hipGraph_t gr = nullptr;
       hipError_t e = hipStreamEndCapture(c->stream, &gr);
+      c->prof = prof;
       if (r1) {
         if (gr) (void)hipGraphDestroy(gr);
         return r1;
@@ -922,14 +922,24 @@ int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int n
       c->g_stime = stime;
       c->g_version = c->version;
     }
-    while (left >= 2) {
+  }
+  int it = iiter0, left = nsteps;
+  while (left > 0) {
+    const int k = it - iiter0;  // timed steps: k = 0, every, 2 every, ...
+    const bool t0 = sampling && k % every == 0, t1 = sampling && (k + 1) % every == 0;
+    if (graph && left >= 2 && it % 2 == 0 && !t0 && !t1) {
       HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
       it += 2;
       left -= 2;
+      continue;
     }
+    if (sampling) c->prof = t0;
+    rc = fall3d_ads_step(c, dt, stime, it);
+    if (sampling) c->prof = true;
+    if (rc) return rc;
+    ++it;
+    --left;
   }
-  for (; left > 0; --left, ++it)
-    if ((rc = fall3d_ads_step(c, dt, stime, it))) return rc;
   return FALL3D_ADS_OK;
 }
 
@@ -1150,7 +1160,9 @@ int fall3d_ads_set_tiling(fall3d_ads_ctx *c, int dir, int cells_per_thread, int 
 int fall3d_ads_set_profiling(fall3d_ads_ctx *c, int on) {
   if (!c) return FALL3D_ADS_EINVAL;
   drain_events(c);
+  if (on < 0) return fail(c, FALL3D_ADS_EINVAL, "profiling interval must be >= 0");
   c->prof = on != 0;
+  c->prof_every = on;
   for (int k = 0; k < KID_COUNT; ++k) {
     c->kt_ms[k] = 0.0;
     c->kt_n[k] = 0;

@@ -104,7 +104,8 @@ struct fall3d_ads_ctx {
   int version = 0;  // bumped whenever launch parameters of a step change
 
   // profiling
-  bool prof = false;
+  bool prof = false;    // the step being issued records per-kernel events
+  int prof_every = 0;   // fall3d_ads_run times one step in prof_every (0: none)
   std::vector<f3d::host::PendingEvent> pending;
   std::vector<hipEvent_t> evpool;
   double kt_ms[f3d::host::KID_COUNT] = {0, 0, 0, 0, 0};

@@ -199,7 +199,11 @@ int fall3d_ads_get_tiling(fall3d_ads_ctx *ctx, int dir, int *cells_per_thread, i
  * fall3d_ads_get_tiling) and best_ms (3 doubles) may be NULL. */
 int fall3d_ads_autotune(fall3d_ads_ctx *ctx, double dt, double stime, int reps, int tiling[3], double best_ms[3]);
 
-/* Per-kernel timing with HIP events on the context stream.
+/* Per-kernel timing with HIP events on the context stream; resets the totals.
+ * on = 0: off; 1: every step (fall3d_ads_run then launches directly, no
+ * graph); N > 1: fall3d_ads_run times the steps iiter0, iiter0 + N, ...
+ * (launched directly) and replays its graph in between, so the events cost
+ * one step in N. Single fall3d_ads_step calls are timed whenever on != 0.
  * kernel ids: 0 source, 1 sweep x, 2 sweep y, 3 sweep z, 4 reduce. */
 int fall3d_ads_set_profiling(fall3d_ads_ctx *ctx, int on);
 int fall3d_ads_get_kernel_time(fall3d_ads_ctx *ctx, int kernel, double *total_ms, long long *launches);

@@ -232,6 +232,36 @@ def test_deterministic_and_tiling_invariant():
         np.testing.assert_allclose(led, outs[0][1], rtol=1e-12, atol=1e-12 * np.abs(outs[0][1]).max())
 
 
+def test_graph_replays_and_sampled_timing_match_direct_steps():
+    """fall3d_ads_run with graph replays (from an even and an odd first
+    iteration) and with per-kernel events on one step in 4 gives the same
+    bits as direct launches; the sampled run times exactly the steps
+    iiter0, iiter0 + 4, ..."""
+    p = operational_problem(40, 36, 20, 3, seed=5)
+    outs = []
+    for mode in ("direct", "graph", "graph_odd", "sampled"):
+        g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+        g.set_problem(p)
+        dt = g.time_step(0.0, 2, 0.9)
+        g.set_graph(mode != "direct")
+        if mode == "graph_odd":
+            g.run(dt, 5, 0)
+            g.run(dt, 8, 5)
+        else:
+            g.set_profiling(4 if mode == "sampled" else 0)
+            g.run(dt, 13, 0)
+        if mode == "sampled":
+            for k in ("sweep_x", "sweep_y", "sweep_z", "reduce"):
+                ms, n = g.kernel_time(k)
+                assert n == 4 and ms > 0.0, (k, n, ms)
+            g.set_profiling(0)
+        outs.append((g.get_state()[0], g.budget()["ledger"]))
+        g.release()
+    for c, led in outs[1:]:
+        assert np.array_equal(c, outs[0][0])
+        assert np.array_equal(led, outs[0][1])
+
+
 @pytest.mark.parametrize("dims", [(40, 36, 20, 3), (70, 20, 3, 4), (33, 9, 2, 5), (65, 4, 2, 2)])
 def test_stream_bins_per_thread_invariant(dims):
     """y/z streaming sweeps with two bins per thread give the concentration
