@@ -175,6 +175,8 @@ def bench_learned(args):
     gc = ads.ADS(pc.nx, pc.ny, pc.nz, nb, "minmod", "rk", 8, "fp32")
     gc.set_problem(pc)
     dt = min(gc.time_step(0.0, 2, 0.9), 2.0 * gf.time_step(0.0, 2, 0.9))
+    # both runs with their autotuned tile shapes (the autotuner restores the state)
+    tiles = {"fine": list(gf.autotune(0.5 * dt)[0]), "coarse": list(gc.autotune(dt)[0])}
     gc.set_learned(net, 1e-10)
     k = 20
     for g, d, n in ((gf, 0.5 * dt, 2 * k), (gc, dt, k)):
@@ -201,7 +203,8 @@ def bench_learned(args):
                    "training_grid": list(tr), "training_rows": rows, "collect_s": round(t1 - t0, 1),
                    "train_s": round(t2 - t1, 2), "loss_first": float(loss[:10].mean()),
                    "loss_last": float(loss[-50:].mean()), "eval_coarse_steps": nsteps, "errors": res,
-                   "fine_plain_gcbs": round(thr_fine, 3), "speedup_vs_fine": round(thr_learned / thr_fine, 3)},
+                   "fine_plain_gcbs": round(thr_fine, 3), "speedup_vs_fine": round(thr_learned / thr_fine, 3),
+                   "autotuned_tiling": tiles},
         "roofline": None, "cpu_baseline": None,
     }))
     gf.release()
@@ -279,7 +282,8 @@ def main():
         try:
             tr = json.load(open(args.traffic))
             ent = tr.get(f"{args.state}:{dom}") or tr.get(dom)
-            if ent:
+            # measured for one workload: only when this launch moves the same algorithmic bytes
+            if ent and abs(ent.get("algorithmic_bytes", -1) - alg_bytes) <= 1e-6 * alg_bytes:
                 traffic = ent.get("bytes_per_launch")
         except Exception:
             traffic = None

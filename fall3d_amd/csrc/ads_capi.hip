@@ -93,29 +93,44 @@ Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
   Tiling t;
   const int n = axis_n(c, dir);
   if (dir == 0) {
-    // one wave per line (several short lines per wave), C cells per lane,
-    // all bins of a line in the same wave unless the grid is too small
-    int C = 16;
+    // C cells per lane, S = n/C + 1 segments per line. Up to 64 segments:
+    // one wave per line (several short lines per wave); up to 256: wpl waves
+    // of one workgroup per line, trading their edge cells through LDS. All
+    // bins of a line in the same wave(s) unless the grid is too small.
+    // Default C: the smallest with one wave per line if that is at most 8
+    // cells per lane (register pressure grows with C), else 6.
+    int C = 6;
     if (c->user_cpt[0]) {
       C = c->user_cpt[0];
     } else {
       for (int cc : kCpt) {
-        if (n / cc + 1 <= 64) { C = cc; break; }
+        if (n / cc + 1 <= 64) {
+          if (cc <= 8) C = cc;
+          break;
+        }
       }
     }
     t.cpt = C;
     t.S = n / C + 1;  // the last segment holds face n+1 (possibly no cell)
-    if (t.S > 64) t.S = 0;  // unsupported; caught by the caller
-    t.block = 256;
-    const int lpw = t.S ? 64 / t.S : 1;
-    t.TX = 4 * lpw;  // lines per workgroup
+    t.wpl = (t.S + 63) / 64;
+    if (t.wpl > 4) t.S = 0;  // unsupported; caught by the caller
+    int lpb;  // lines per workgroup
+    if (t.wpl <= 1) {
+      t.wpl = 1;
+      t.block = 256;
+      lpb = 4 * (t.S ? 64 / t.S : 1);
+    } else {
+      lpb = 4 / t.wpl;
+      t.block = 64 * t.wpl * lpb;
+    }
+    t.TX = lpb;
     const long nlines = (long)c->ny * c->nz;
     t.gx = (int)((nlines + t.TX - 1) / t.TX);
     int groups = 1;
     if (c->user_bpg[0] > 0) {
       groups = (c->nb + c->user_bpg[0] - 1) / c->user_bpg[0];
     } else {
-      const long waves = (long)t.gx * 4;
+      const long waves = (long)t.gx * (t.block / 64);
       const long target = 4096;
       if (waves < target) groups = (int)std::min<long>(c->nb, (target + waves - 1) / std::max<long>(waves, 1));
     }
@@ -145,7 +160,7 @@ int refresh_tiling(fall3d_ads_ctx *c) {
   for (int d = 0; d < 3; ++d) {
     Tiling t = choose_tiling(c, d);
     if (d == 0 && (t.S <= 0 || t.cpt < 2 || !sweep_cpt_supported(t.cpt)))
-      return fail(c, FALL3D_ADS_EINVAL, "no supported x tiling (nx=%d cpt=%d; nx < 64 cpt is required)",
+      return fail(c, FALL3D_ADS_EINVAL, "no supported x tiling (nx=%d cpt=%d; nx < 256 cpt is required)",
                   axis_n(c, d), t.cpt);
     c->tl[d] = t;
     const size_t need = (d == 0) ? (size_t)t.gx * t.groups : (size_t)t.gx;
@@ -188,6 +203,7 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
   a.nz = c->nz;
   a.n = axis_n(c, dir);
   a.S = t.S;
+  a.wpl = t.wpl;
   a.TX = t.TX;
   a.ntrans = t.ntrans;
   a.ntrans2 = t.groups;
@@ -1046,7 +1062,7 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
     std::vector<int> cand;
     if (d == 0) {
       for (int i = 0; i < kSweepCptCount; ++i)
-        if (c->nx / kSweepCpt[i] + 1 <= 64) cand.push_back(kSweepCpt[i]);
+        if (c->nx / kSweepCpt[i] + 1 <= 256) cand.push_back(kSweepCpt[i]);  // up to 4 waves per line
     } else {
       for (int b = 1; b <= (c->comp ? 1 : 2); ++b)
         for (int g = b; g <= std::min(c->nb, 8); ++g) {

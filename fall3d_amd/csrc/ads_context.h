@@ -23,6 +23,7 @@ enum { ACC_FLUX = 0, ACC_LEDGER = 6, ACC_INJECTED = 12, ACC_SCRATCH = 13, ACC_LE
 struct Tiling {
   int cpt = 0;      // cells per lane (x sweep) / bins per thread (y, z sweeps)
   int S = 0;        // segments per line
+  int wpl = 1;      // x sweep: waves per line
   int TX = 0;       // lines per workgroup (y, z) / lines per workgroup (x)
   int block = 0;    // threads per workgroup
   int gx = 0;       // workgroups along grid.x

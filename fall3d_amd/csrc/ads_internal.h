@@ -25,6 +25,7 @@ struct SweepArgs {
   int nx, ny, nz;         // interior extents
   int n;                  // cells along the sweep axis
   int S;                  // segments (threads) per line
+  int wpl;                // x sweep: waves per line (1: a wave holds one or more whole lines)
   int TX;                 // lines per workgroup (x-columns for y/z sweeps)
   int ntrans;             // x tiles of 64 columns (y, z sweeps)
   int ntrans2;            // bin groups (y, z sweeps; blockIdx.x = (line, tile) * groups + group)

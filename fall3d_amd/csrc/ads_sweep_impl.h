@@ -690,16 +690,39 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   static_assert(C >= 2, "segments need two edge cells");
   // Segmentation: S = n/C + 1 segments of C cells, so that face n+1 always
   // lies inside a segment (faces 0..C-1 of each lane are computed locally).
+  // S <= 64: a wave holds 64/S whole lines. S > 64: wpl waves of the
+  // workgroup hold one line, segment = 64 (wave in line) + lane; the lanes
+  // at wave edges trade their cells and fluxes through LDS, with a
+  // workgroup barrier per exchange (wpl is uniform, so the branches are).
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int S = a.S, n = a.n;
+  const int S = a.S, n = a.n, wpl = a.wpl;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int lpw = 64 / S;  // lines per wave
-  const int lslot = lane / S, seg = lane % S;
-  const int line = (bx * (blockDim.x >> 6) + wave) * lpw + lslot;
-  const bool act = (lslot < lpw) && (line < ny * nz);
+  int seg, line, wil = 0;
+  bool act;
+  if (wpl == 1) {
+    const int lpw = 64 / S;  // lines per wave
+    const int lslot = lane / S;
+    seg = lane % S;
+    line = (bx * (blockDim.x >> 6) + wave) * lpw + lslot;
+    act = (lslot < lpw) && (line < ny * nz);
+  } else {
+    wil = wave % wpl;
+    seg = wil * 64 + lane;
+    line = bx * ((blockDim.x >> 6) / wpl) + wave / wpl;
+    act = (seg < S) && (line < ny * nz);
+  }
   const int j = act ? line % ny + 1 : 1;
   const int k = act ? line / ny + 1 : 1;
-  const int s0 = seg * C + 1;
+  // idle segments of a multi-wave line read line (1, 1) from its start
+  const int s0 = (act || wpl == 1) ? seg * C + 1 : 1;
+  // edge exchange between the waves of a line: per wave, the left edge's
+  // y[C-2], y[C-1] (slots 0, 1), the right edge's y[0], G0 and gh[0]
+  // (slots 2, 3, 4) and the left edge's unmodified last cell (slot 5); each
+  // slot is written once per exchange phase, and a phase's reads finish
+  // before the next phase's barrier
+  __shared__ float xch[16][6];
+  const bool xl = wpl > 1 && lane == 0 && wil > 0;         // takes from the wave to its left
+  const bool xr = wpl > 1 && lane == 63 && wil < wpl - 1;  // takes from the wave to its right
   const bool first = (seg == 0) || !act;
   const bool last = (seg == S - 1) || !act;
   const int qn = n - s0;  // local index of cell n in the last segment
@@ -780,7 +803,12 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
     // Cell n is in the last segment, or (when that segment starts at n+1)
     // the last cell of the segment to its left.
     float gL = 0.0f, gR = 0.0f;
-    const float cleft = __shfl(cn[C - 1], lsrc, 64);
+    float cleft = __shfl(cn[C - 1], lsrc, 64);
+    if (wpl > 1) {
+      if (lane == 63) xch[wave][5] = cn[C - 1];
+      __syncthreads();
+      if (xl) cleft = xch[wave - 1][5];
+    }
     if (first) gL = (vbeg > 0.0f) ? (2.0f * 0.0f - cn[0]) : cn[0];
     if (last) {
       float cl = (qn < 0) ? cleft : cn[0];
@@ -825,6 +853,19 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       float yl2 = __shfl(y[C - 2], lsrc, 64);
       float yl1 = __shfl(y[C - 1], lsrc, 64);
       float yr0 = __shfl(y[0], rsrc, 64);
+      if (wpl > 1) {
+        if (lane == 63) {
+          xch[wave][0] = y[C - 2];
+          xch[wave][1] = y[C - 1];
+        }
+        if (lane == 0) xch[wave][2] = y[0];
+        __syncthreads();
+        if (xl) {
+          yl2 = xch[wave - 1][0];
+          yl1 = xch[wave - 1][1];
+        }
+        if (xr) yr0 = xch[wave + 1][2];
+      }
       if (first) {
         yl2 = gL;
         yl1 = gL;
@@ -871,10 +912,19 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       // Face C is the right neighbour's face 0 (same inputs, same arithmetic:
       // bit-identical). The last segment's face C lies beyond n+1 and only
       // feeds frozen ghost positions (zero multipliers).
-      const float GC = __shfl(G0, rsrc, 64);
+      float GC = __shfl(G0, rsrc, 64);
+      if (wpl > 1) {
+        if (lane == 0) {
+          xch[wave][3] = G0;
+          if (last_stage) xch[wave][4] = gh[0];
+        }
+        __syncthreads();
+        if (xr) GC = xch[wave + 1][3];
+      }
       if (!last_stage) y[C - 1] = fmaf((st == 2) ? db[C - 1] : hb[C - 1], G_prev - GC, cn[C - 1]);
     }
     gh[C] = __shfl(gh[0], rsrc, 64);
+    if (xr) gh[C] = xch[wave + 1][4];
 
     if constexpr (FOUT) {  // face mass flux per unit area: faces s0 .. s0+C-1, and n+1
       float *fo = a.fout + (size_t)(nx + 3) * ny * nz * b + vbase;

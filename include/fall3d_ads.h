@@ -185,9 +185,11 @@ int fall3d_ads_set_stream(fall3d_ads_ctx *ctx, void *stream);
 int fall3d_ads_synchronize(fall3d_ads_ctx *ctx);
 
 /* Tuning of the sweep of axis dir (0 x, 1 y, 2 z), 0 = automatic:
- * cells_per_thread is the x sweep's cells per lane, and the y/z sweeps' bins
- * per thread (1 or 2; 2 not with the compensated state); columns: x-columns
- * per workgroup; bins_per_group: bins per workgroup. */
+ * cells_per_thread is the x sweep's cells per lane (2, 3, 4, 6, 8, 12, 16
+ * or 20; an x line of nx cells takes nx/C + 1 lanes, at most 256: one wave
+ * holds one or more lines, or 2-4 waves of a workgroup hold one), and the
+ * y/z sweeps' bins per thread (1 or 2; 2 not with the compensated state);
+ * columns: x-columns per workgroup; bins_per_group: bins per workgroup. */
 int fall3d_ads_set_tiling(fall3d_ads_ctx *ctx, int dir, int cells_per_thread, int columns, int bins_per_group);
 /* The tile shape in use for axis dir: cells (x) or bins (y/z) per thread,
  * bins per workgroup. */

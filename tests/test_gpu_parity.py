@@ -232,6 +232,25 @@ def test_deterministic_and_tiling_invariant():
         np.testing.assert_allclose(led, outs[0][1], rtol=1e-12, atol=1e-12 * np.abs(outs[0][1]).max())
 
 
+@pytest.mark.parametrize("state", ["fp32", "fp32x2"])
+def test_multiwave_x_lines_bit_identical(state):
+    """Lines held by 2 or 3 waves (edge cells traded through LDS) give the
+    same bits as lines held by one wave."""
+    p = operational_problem(200, 9, 6, 2, seed=8)
+    outs = []
+    for cpt in (4, 2, 3):  # 51 segments: one wave; 101 and 67: two waves
+        g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, state)
+        g.set_problem(p)
+        g.set_tiling(0, cells_per_thread=cpt)
+        dt = g.time_step(0.0, 2, 0.9)
+        g.run(dt, 5, 0)
+        outs.append(g.get_state())
+        g.release()
+    for c, a in outs[1:]:
+        assert np.array_equal(c, outs[0][0])
+        assert np.array_equal(a, outs[0][1])
+
+
 def test_graph_replays_and_sampled_timing_match_direct_steps():
     """fall3d_ads_run with graph replays (from an even and an odd first
     iteration) and with per-kernel events on one step in 4 gives the same
@@ -319,11 +338,14 @@ def test_operational_full_size_two_steps():
         g.release()
 
 
-@pytest.mark.parametrize("nx,cpt", [(12, 4), (12, 3), (13, 4), (16, 16), (64, 2), (63, 2), (40, 20)])
+@pytest.mark.parametrize("nx,cpt", [(12, 4), (12, 3), (13, 4), (16, 16), (64, 2), (63, 2), (40, 20),
+                                    (128, 2), (200, 2), (383, 3), (300, 2), (500, 2), (1024, 6)])
 def test_x_segmentation_edge_cases(nx, cpt):
     """x sweep with every segmentation case: cell n at the end of a segment
     (the last segment then holds only ghosts and face n+1), in its middle,
-    one segment per line, and 64-lane lines; against the oracle."""
+    one segment per line, and 64-lane lines; lines over 2, 3 and 4 waves
+    (nx/cpt + 1 segments > 64), including a last wave whose only segment
+    starts at face n+1; against the oracle."""
     print(f"case nx={nx} cpt={cpt}", flush=True)
     p = random_problem(nx, 7, 5, 2, seed=nx + cpt)
     o = Oracle(p, 1, 2)
