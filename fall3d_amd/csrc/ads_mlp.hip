@@ -219,13 +219,25 @@ __global__ void __launch_bounds__(256) k_mlp_grad(const PackedWeights *__restric
   }
 }
 
-// grad[p] = sum over waves of part[w][p] (fixed order), p <= NPARAM
-__global__ void k_mlp_reduce(const float *__restrict__ part, int nwave, float *__restrict__ grad) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > MLP_NPARAM) return;
+// grad[p] = sum over waves of part[w][p], p <= NPARAM, in a fixed order: a
+// workgroup takes 64 parameters; its 16 waves sum the partials w = r,
+// r + 16, r + 32, ... (r = wave), then wave 0 adds the 16 sums in order r.
+constexpr int kReduceSlots = 16;
+__global__ void __launch_bounds__(64 * kReduceSlots) k_mlp_reduce(const float *__restrict__ part, int nwave,
+                                                                  float *__restrict__ grad) {
+  __shared__ float sums[kReduceSlots][64];
+  const int l = threadIdx.x & 63, r = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + l;
   float s = 0.0f;
-  for (int w = 0; w < nwave; ++w) s += part[(size_t)w * (MLP_NPARAM + 1) + p];
-  grad[p] = s;
+  if (p <= MLP_NPARAM)
+    for (int w = r; w < nwave; w += kReduceSlots) s += part[(size_t)w * (MLP_NPARAM + 1) + p];
+  sums[r][l] = s;
+  __syncthreads();
+  if (r == 0 && p <= MLP_NPARAM) {
+    float t = sums[0][l];
+    for (int q = 1; q < kReduceSlots; ++q) t += sums[q][l];
+    grad[p] = t;
+  }
 }
 
 // Adam (Kingma & Ba 2015, Algorithm 1) on the FP32 master parameters
@@ -270,7 +282,8 @@ hipError_t mlp_grad(const PackedWeights *w, const uint16_t *x, const float *t, c
                     float scale, float *part, int nwaves, float *grad, hipStream_t st) {
   (void)hipGetLastError();
   hipLaunchKernelGGL(k_mlp_grad, dim3(nwaves / 4), dim3(256), 0, st, w, x, t, idx, n, scale, part);
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 1 + 255) / 256), dim3(256), 0, st, part, nwaves, grad);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 1 + 63) / 64), dim3(64 * kReduceSlots), 0, st, part, nwaves,
+                     grad);
   return hipGetLastError();
 }
 
