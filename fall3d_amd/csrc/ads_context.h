@@ -119,7 +119,8 @@ struct fall3d_ads_ctx {
   float *dphi[2] = {nullptr, nullptr};   // its face corrections of the current step
   int *lact[2] = {nullptr, nullptr};     // active 16-face tiles of each axis (compacted list)
   unsigned char *lflag[2] = {nullptr, nullptr};  // per tile: 1 = active this step
-  unsigned *lcount = nullptr;            // their counts (2)
+  int *lseg = nullptr;                   // 64-cell row segments next to an active tile (compacted list)
+  unsigned *lcount = nullptr;            // counts of the three lists
   double learned_smin = 0.0;
   double learned_gain = 1.0;
   double learned_flux_limit = 0.0;       // |dPhi| <= limit * |plain face flux| (0: off)

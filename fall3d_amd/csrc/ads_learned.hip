@@ -380,89 +380,100 @@ __device__ __forceinline__ float limited(const float *__restrict__ d, const floa
   const float m = lim * fabsf(phi[i]);
   return fminf(fmaxf(v, -m), m);
 }
-// Each wave takes a contiguous range of 64-cell segments of x rows (j, k, b),
-// nseg segments per row, nsegs of them in all (the host keeps that below
-// 2^31), eight segments at a time. The tile flags that a segment needs are
-// wave-uniform: the 4 x tiles of its faces 64 sx + 2 .. 64 sx + 65 (one
+// Tile flags a 64-cell segment sx of cell row (j, kb) needs, kb = (k - 1) +
+// nz b, as words: the 4 x tiles of its faces 64 sx + 2 .. 64 sx + 65 (one
 // 32-bit word, tile rows being whole words), the x tile before them (face
-// 64 sx + 1), and the 4 y tiles of the face rows j and j + 1. A segment whose
-// flags are all zero has no correction and is skipped; in the others a cell
-// none of whose four faces lies in an active tile is left untouched.
+// 64 sx + 1), and the 4 y tiles of the face rows j and j + 1 (zero where the
+// row or word does not exist).
+struct SegFlags {
+  unsigned wx, wp, wy0, wy1;
+};
+__device__ __forceinline__ SegFlags seg_flags(const unsigned char *__restrict__ flx,
+                                              const unsigned char *__restrict__ fly, int ntx, int nty, int ny,
+                                              int j, int kb, int sx) {
+  const unsigned *__restrict__ wfx = reinterpret_cast<const unsigned *>(flx);
+  const unsigned *__restrict__ wfy = reinterpret_cast<const unsigned *>(fly);
+  const size_t rx = (size_t)(j - 1) + (size_t)ny * kb;  // x-face row of these cells
+  SegFlags f;
+  f.wx = (sx < (ntx >> 2)) ? wfx[rx * (ntx >> 2) + sx] : 0u;
+  f.wp = (sx > 0) ? flx[rx * ntx + 4 * sx - 1] : 0u;
+  f.wy0 = (j >= 2) ? wfy[((size_t)(j - 2) + (size_t)(ny - 1) * kb) * (nty >> 2) + sx] : 0u;
+  f.wy1 = (j + 1 <= ny) ? wfy[((size_t)(j - 1) + (size_t)(ny - 1) * kb) * (nty >> 2) + sx] : 0u;
+  return f;
+}
+
+// List of the 64-cell row segments with a face in an active tile (one thread
+// per segment, one atomic per wave). nsegs = nseg * ny * nz * nb < 2^31.
+__global__ void __launch_bounds__(256) k_touched_segments(const unsigned char *__restrict__ flx,
+                                                          const unsigned char *__restrict__ fly, int nx, int ny,
+                                                          int nz, int nb, int nseg, int *__restrict__ list,
+                                                          unsigned *__restrict__ count) {
+  int ntx, nty;
+  long rows_unused;
+  face_space(0, nx, ny, nz, nb, ntx, rows_unused);
+  face_space(1, nx, ny, nz, nb, nty, rows_unused);
+  const unsigned nsegs = (unsigned)nseg * (unsigned)ny * (unsigned)nz * (unsigned)nb;
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool t = false;
+  if (g < nsegs) {
+    const unsigned row = g / (unsigned)nseg;  // (j - 1) + ny kb
+    const int sx = (int)(g - row * (unsigned)nseg);
+    const unsigned kb = row / (unsigned)ny;
+    const int j = (int)(row - kb * (unsigned)ny) + 1;
+    const SegFlags f = seg_flags(flx, fly, ntx, nty, ny, j, (int)kb, sx);
+    t = (f.wx | f.wp | f.wy0 | f.wy1) != 0u;
+  }
+  const uint64_t m = __ballot(t);
+  if (m == 0) return;  // uniform
+  const int l = threadIdx.x & 63;
+  unsigned base = 0;
+  if (l == 0) base = atomicAdd(count, (unsigned)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (t) list[base + __popcll(m & (((uint64_t)1 << l) - 1))] = (int)g;
+}
+
+// Each wave takes listed segments (uniform), one cell per lane; a cell none
+// of whose four faces lies in an active tile is left untouched: its
+// correction is zero. Every cell is updated by one thread, so the result
+// does not depend on the order of the list.
 __global__ void __launch_bounds__(256) k_apply_corrections(
     float *__restrict__ c, const float *__restrict__ dx, const float *__restrict__ dy, const float *__restrict__ px,
     const float *__restrict__ py, const unsigned char *__restrict__ flx, const unsigned char *__restrict__ fly,
-    float lim, int nx, int ny, int nz, int nb, int nseg, unsigned per_wave, const float *__restrict__ idxb_x,
-    const float *__restrict__ idxb_y) {
+    float lim, int nx, int ny, int nz, int nb, int nseg, const int *__restrict__ list,
+    const unsigned *__restrict__ count, const float *__restrict__ idxb_x, const float *__restrict__ idxb_y) {
   int ntx, nty;  // tiles per face row of each axis (multiples of 4)
   long rows_unused;
   face_space(0, nx, ny, nz, nb, ntx, rows_unused);
   face_space(1, nx, ny, nz, nb, nty, rows_unused);
-  const int nwx = ntx >> 2, nwy = nty >> 2;  // flag words per face row
-  const unsigned *__restrict__ wfx = reinterpret_cast<const unsigned *>(flx);
-  const unsigned *__restrict__ wfy = reinterpret_cast<const unsigned *>(fly);
-  const unsigned nsegs = (unsigned)nseg * (unsigned)ny * (unsigned)nz * (unsigned)nb;
-  const unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const unsigned s0 = wave * per_wave;
-  if (s0 >= nsegs) return;
-  const unsigned s1 = min(nsegs, s0 + per_wave);
+  const unsigned na = *count;
+  const unsigned nwave = gridDim.x * (blockDim.x >> 6);
   const int l = threadIdx.x & 63;
-  // segment position: sx along the row, row = (j - 1) + ny kb, kb = (k - 1) + nz b
-  const unsigned row0 = s0 / (unsigned)nseg;
-  int sx = (int)(s0 - row0 * (unsigned)nseg);
-  const unsigned kb0 = row0 / (unsigned)ny;
-  int j = (int)(row0 - kb0 * (unsigned)ny) + 1;
-  int kb = (int)kb0;
-  constexpr int U = 8;
-  for (unsigned sg = s0; sg < s1; sg += U) {
-    unsigned wx[U], wp[U], wy0[U], wy1[U];
-    int sj[U], skb[U], ssx[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {  // flag loads of eight segments (invalid ones read word 0)
-      const bool v = sg + u < s1;
-      const size_t rx = (size_t)(j - 1) + (size_t)ny * kb;  // x-face row of these cells
-      const bool vx = v && sx < nwx, vp = v && sx > 0, vy0 = v && j >= 2, vy1 = v && j + 1 <= ny;
-      wx[u] = wfx[vx ? rx * nwx + sx : 0];
-      wp[u] = flx[vp ? rx * ntx + 4 * sx - 1 : 0];
-      wy0[u] = wfy[vy0 ? ((size_t)(j - 2) + (size_t)(ny - 1) * kb) * nwy + sx : 0];
-      wy1[u] = wfy[vy1 ? ((size_t)(j - 1) + (size_t)(ny - 1) * kb) * nwy + sx : 0];
-      wx[u] = vx ? wx[u] : 0u;
-      wp[u] = vp ? wp[u] : 0u;
-      wy0[u] = vy0 ? wy0[u] : 0u;
-      wy1[u] = vy1 ? wy1[u] : 0u;
-      sj[u] = j;
-      skb[u] = kb;
-      ssx[u] = sx;
-      if (++sx == nseg) {  // next row
-        sx = 0;
-        if (++j > ny) {
-          j = 1;
-          ++kb;
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if ((wx[u] | wp[u] | wy0[u] | wy1[u]) == 0u) continue;  // uniform
-      const int i = ssx[u] * 64 + l + 1, jj = sj[u];
-      if (i > nx) continue;
-      // tile of face i: the word's tile ((l - 1) >> 4) (lane 0: the tile
-      // before); of face i + 1: tile l >> 4; of the y faces: tile l >> 4
-      const uint64_t X = (uint64_t)wp[u] | ((uint64_t)wx[u] << 8);
-      const int tl0 = (l == 0) ? 0 : ((l - 1) >> 4) + 1, tl1 = (l >> 4) + 1;
-      const bool ox0 = i >= 2 && ((X >> (8 * tl0)) & 1);
-      const bool ox1 = i + 1 <= nx && ((X >> (8 * tl1)) & 1);
-      const bool oy0 = (wy0[u] >> (8 * (l >> 4))) & 1;
-      const bool oy1 = (wy1[u] >> (8 * (l >> 4))) & 1;
-      if (!(ox0 || ox1 || oy0 || oy1)) continue;
-      const size_t kbs = (size_t)skb[u];
-      const int k = skb[u] % nz + 1, b = skb[u] / nz;
-      const size_t fx = (size_t)i + (size_t)(nx + 3) * ((size_t)(jj - 1) + (size_t)ny * kbs);
-      const size_t fy = (size_t)(i - 1) + (size_t)nx * ((size_t)jj + (size_t)(ny + 3) * kbs);
-      const float d = (limited(dx, px, fx + 1, lim, ox1) - limited(dx, px, fx, lim, ox0)) * idxb_x[i] +
-                      (limited(dy, py, fy + nx, lim, oy1) - limited(dy, py, fy, lim, oy0)) * idxb_y[jj];
-      const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(jj + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
-      c[ci] -= d;
-    }
+  for (unsigned q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < na; q += nwave) {
+    const unsigned sg = (unsigned)list[q];
+    const unsigned row = sg / (unsigned)nseg;  // (j - 1) + ny kb
+    const int sx = (int)(sg - row * (unsigned)nseg);
+    const int kb = (int)(row / (unsigned)ny);
+    const int j = (int)(row - (unsigned)kb * (unsigned)ny) + 1;
+    const SegFlags fl = seg_flags(flx, fly, ntx, nty, ny, j, kb, sx);
+    const int i = sx * 64 + l + 1;
+    if (i > nx) continue;
+    // tile of face i: the word's tile ((l - 1) >> 4) (lane 0: the tile
+    // before); of face i + 1: tile l >> 4; of the y faces: tile l >> 4
+    const uint64_t X = (uint64_t)fl.wp | ((uint64_t)fl.wx << 8);
+    const int tl0 = (l == 0) ? 0 : ((l - 1) >> 4) + 1, tl1 = (l >> 4) + 1;
+    const bool ox0 = i >= 2 && ((X >> (8 * tl0)) & 1);
+    const bool ox1 = i + 1 <= nx && ((X >> (8 * tl1)) & 1);
+    const bool oy0 = (fl.wy0 >> (8 * (l >> 4))) & 1;
+    const bool oy1 = (fl.wy1 >> (8 * (l >> 4))) & 1;
+    if (!(ox0 || ox1 || oy0 || oy1)) continue;
+    const int k = kb % nz + 1, b = kb / nz;
+    const size_t kbs = (size_t)kb;
+    const size_t fx = (size_t)i + (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * kbs);
+    const size_t fy = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * kbs);
+    const float d = (limited(dx, px, fx + 1, lim, ox1) - limited(dx, px, fx, lim, ox0)) * idxb_x[i] +
+                    (limited(dy, py, fy + nx, lim, oy1) - limited(dy, py, fy, lim, oy0)) * idxb_y[j];
+    const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
+    c[ci] -= d;
   }
 }
 
@@ -694,7 +705,7 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "learned threshold: %s", hipGetErrorString(e));
   }
   // a fill kernel, not a memset: the step may be captured in a graph
-  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, c->stream, c->lcount, 2);
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, c->stream, c->lcount, 3);
   for (int axis = 0; axis < 2; ++axis) {
     FaceArgs a = face_args(c, axis, dt, stime, rel ? 0.0 : c->learned_smin);
     if (rel) a.smin_dev = c->lthr;
@@ -714,12 +725,13 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
 int learned_post_step(fall3d_ads_ctx *c) {
   const int nseg = (c->nx + 63) / 64;
   const long nsegs = (long)c->ny * c->nz * c->nb * nseg;
-  const long per_wave = std::max<long>(8, (nsegs + 65535) / 65536);  // at most 65536 waves
-  const unsigned blocks = (unsigned)((nsegs + 4 * per_wave - 1) / (4 * per_wave));
   const float lim = (float)c->learned_flux_limit;
+  hipLaunchKernelGGL(k_touched_segments, dim3((unsigned)((nsegs + 255) / 256)), dim3(256), 0, c->stream, c->lflag[0],
+                     c->lflag[1], c->nx, c->ny, c->nz, c->nb, nseg, c->lseg, c->lcount + 2);
+  const unsigned blocks = (unsigned)std::min<long>((nsegs + 3) / 4, 8192);
   hipLaunchKernelGGL(k_apply_corrections, dim3(blocks), dim3(256), 0, c->stream, c->chi, c->dphi[0], c->dphi[1],
                      lim > 0.0f ? c->fout[0] : c->dphi[0], lim > 0.0f ? c->fout[1] : c->dphi[1], c->lflag[0],
-                     c->lflag[1], lim, c->nx, c->ny, c->nz, c->nb, nseg, (unsigned)per_wave, c->idxb[0],
+                     c->lflag[1], lim, c->nx, c->ny, c->nz, c->nb, nseg, c->lseg, c->lcount + 2, c->idxb[0],
                      c->idxb[1]);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "apply corrections: %s", hipGetErrorString(e));
@@ -740,6 +752,8 @@ void learned_release(fall3d_ads_ctx *c) {
     c->lact[a] = nullptr;
     c->lflag[a] = nullptr;
   }
+  dfree(c->lseg);
+  c->lseg = nullptr;
   dfree(c->lcount);
   c->lcount = nullptr;
   c->learned = nullptr;
@@ -970,7 +984,8 @@ int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, doub
       return fail(c, FALL3D_ADS_EINVAL, "the flux-relative limit needs minmod and RK4");
     if (ntiles(c, 0) > INT_MAX || ntiles(c, 1) > INT_MAX || (long)c->ny * c->nz * c->nb * ((c->nx + 63) / 64) > INT_MAX)
       return fail(c, FALL3D_ADS_EINVAL, "domain too large for the learned correction's tile list");
-    if (!c->lcount) HIPCHK(c, dalloc(&c->lcount, 2));
+    if (!c->lcount) HIPCHK(c, dalloc(&c->lcount, 3));
+    if (!c->lseg) HIPCHK(c, dalloc(&c->lseg, (size_t)c->ny * c->nz * c->nb * ((c->nx + 63) / 64)));
     for (int a = 0; a < 2; ++a) {
       if (!c->dphi[a]) HIPCHK(c, dalloc(&c->dphi[a], nface(c, a)));
       if (!c->lact[a]) HIPCHK(c, dalloc(&c->lact[a], (size_t)ntiles(c, a)));
