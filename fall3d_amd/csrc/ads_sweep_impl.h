@@ -418,10 +418,17 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   // ghosts and drop their stores).
   const int nblk = (tend + kPF - 1 + 15) / 16;
   const int tstart = tend + 1 - 16 * nblk;
-  rnext = itab[tstart + kItabOffset];
+  // z lines are short (60 cells take 80 iterations from tstart): when the
+  // start may move 8 iterations later and still see only ghosts, the first
+  // block runs its second half only. With two groups in flight the group
+  // sets keep their parity (u = 8 reads set 0, as u = 0 does).
+  constexpr bool kHalf = (DIR == 2) && (kGD == 2);
+  const bool half = kHalf && (tstart + 8 <= 2 - kPF);
+  const int tfirst = half ? tstart + 8 : tstart;
+  rnext = itab[tfirst + kItabOffset];
   // the first group's rows are all ghosts (the zeros above become them);
   // the rows of the next kGD-1 groups are fetched here
-  unroll_range<1, kGD>([&](auto K) { fetch(tstart + 4 * (decltype(K)::value - 1), K); });
+  unroll_range<1, kGD>([&](auto K) { fetch(tfirst + 4 * (decltype(K)::value - 1), K); });
 
   // One pipeline step at iteration t (u = (t - tstart) mod 16, a
   // compile-time constant: ring slots are static registers).
@@ -594,7 +601,14 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
     __builtin_amdgcn_sched_barrier(0);
   };
 
-  for (int t0 = tstart; t0 <= tend; t0 += 16)
+  int t0 = tstart;
+  if constexpr (kHalf) {
+    if (half) {
+      unroll_range<8, 16>([&](auto U) { step(t0 + decltype(U)::value, U); });
+      t0 += 16;
+    }
+  }
+  for (; t0 <= tend; t0 += 16)
     unroll_range<0, 16>([&](auto U) { step(t0 + decltype(U)::value, U); });
 
   double rb = 0.0, re = 0.0, lb = 0.0, le = 0.0;
