@@ -132,7 +132,7 @@ def bench_ensemble(args):
 
 def bench_learned(args):
     """Train the flux correction on fine/coarse runs (512x512x40, 4 bins, 32
-    seeds at scale 1) and compare plain coarse, learned coarse and plain fine
+    seeds at scale 1, 8 samples 60 fine steps apart per seed) and compare plain coarse, learned coarse and plain fine
     (1024x768x80, 20 bins, 3 line sources at scale 1) against the fine run of
     the compensated state; throughput in fine-equivalent cell*bin/s."""
     import numpy as np
@@ -148,9 +148,11 @@ def bench_learned(args):
 
     tr = (even(512), even(512), max(8, int(40 * sc)), 4)
     kw_tr = {"dx": 5000.0, "dz": 500.0, "nsources": 2, "puff": True, "vmin": 0.005, "vmax": 5.0}
-    net = ads.Network(capacity=int(60_000_000 * min(1.0, sc * sc) + 1_000_000))
+    net = ads.Network(capacity=int(120_000_000 * min(1.0, sc * sc) + 1_000_000))
     t0 = time.perf_counter()
-    rows = learned.collect(net, range(int(max(4, 32 * min(1.0, sc)))), tr, nwarm=40, nsamples=4, gap=6, **kw_tr)
+    # samples spread over ~900 fine steps of each run: the evaluation runs 1000
+    # fine-equivalent steps, so the rows cover early and late plumes alike
+    rows = learned.collect(net, range(int(max(4, 32 * min(1.0, sc)))), tr, nwarm=40, nsamples=8, gap=60, **kw_tr)
     t1 = time.perf_counter()
     print(f"bench: {rows} training rows in {t1 - t0:.1f} s", file=sys.stderr, flush=True)
     loss = net.train(2000, batch=65536, lr=1e-3, seed=1)
