@@ -119,6 +119,7 @@ struct fall3d_ads_ctx {
   float *dphi[2] = {nullptr, nullptr};   // its face corrections of the current step
   int *lact[2] = {nullptr, nullptr};     // active 16-face tiles of each axis (compacted list)
   unsigned char *lflag[2] = {nullptr, nullptr};  // per tile: 1 = active this step
+  unsigned long long *lbits = nullptr;  // per cell row, bit i-1 of word (i-1)/64: |c(i)| above the threshold
   int *lseg = nullptr;                   // 64-cell row segments next to an active tile (compacted list)
   unsigned *lcount = nullptr;            // counts of the three lists
   double learned_smin = 0.0;

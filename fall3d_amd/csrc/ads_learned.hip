@@ -251,76 +251,122 @@ __device__ __forceinline__ size_t face_index(const FaceArgs &a, int f, int tr, i
 // ---------------------------------------------------------------------------
 // inference: dPhi of every interior face of one axis (others stay zero)
 // ---------------------------------------------------------------------------
-// Activity pass: each wave takes 16 consecutive 64-face groups (4 tiles
-// each; a group never straddles rows), one face per lane. A tile is active
-// when one of its faces has a cell above the threshold; the pass writes one
-// flag per tile and appends the active tiles to the list (one atomic per
-// wave; the order of the list does not matter, every face is computed from
-// its own stencil). Faces of inactive tiles get no correction: the apply pass
-// reads their flag, not their dPhi. The host keeps the tile count below 2^31.
-__global__ void __launch_bounds__(256) k_active_tiles(FaceArgs a, unsigned char *__restrict__ flag,
-                                                      int *__restrict__ list, unsigned *__restrict__ count) {
+// Activity, pass 1: one bit per cell, |c| above max(smin, 1e-30), in words of
+// 64 cells per row (j, k, b): cell i is bit (i-1) & 63 of word (i-1) / 64 of
+// the row's nw words. Each wave takes 16 consecutive words (one cell per
+// lane; a ballot makes the word). The host keeps the word count below 2^31.
+__global__ void __launch_bounds__(256) k_live_bits(FaceArgs a, unsigned long long *__restrict__ bits) {
   const int l = threadIdx.x & 63;
-  int nt;
-  long rows;
-  face_space(a.axis, a.nx, a.ny, a.nz, a.nb, nt, rows);
-  const unsigned ntile = (unsigned)nt * (unsigned)rows;
-  const int ngr = nt >> 2;  // 64-face groups per row
-  const unsigned ngroups = ntile >> 2;
-  const unsigned G0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u;
-  if (G0 >= ngroups) return;
-  const unsigned t0 = 4 * G0;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int nw = (nx + 63) / 64;
+  const unsigned nwords = (unsigned)nw * (unsigned)ny * (unsigned)nz * (unsigned)a.nb;
+  const unsigned w0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16u;
+  if (w0 >= nwords) return;
   const float thr = fmaxf(a.smin_dev ? *a.smin_dev : a.smin, kSmallest);
-  const int nfr = (a.axis == 0) ? a.nx - 1 : a.nx;  // faces per row
-  const int ry = (a.axis == 0) ? a.ny : a.ny - 1;   // rows per level
-  // row = r + ry (k + nz b), all 0-based; the first face's cells are
-  // (1, r + 1) and, along the face normal, the next one
-  const unsigned row0 = G0 / (unsigned)ngr;
-  int xg = (int)(G0 - row0 * (unsigned)ngr);
-  const unsigned kb0 = row0 / (unsigned)ry;
-  int r = (int)(row0 - kb0 * (unsigned)ry);
-  int k = (int)(kb0 % (unsigned)a.nz), b = (int)(kb0 / (unsigned)a.nz);
-  const size_t d2 = (a.axis == 0) ? 1 : (size_t)(a.nx + 4);
-  size_t cb = cix(a, 1, r + 1, k + 1, b);
-  // all 32 loads first (faces past the end read element 0), then the tests
-  float v0[16], v1[16];
+  const unsigned row0 = w0 / (unsigned)nw;
+  int xw = (int)(w0 - row0 * (unsigned)nw);
+  int kb = (int)(row0 / (unsigned)ny);
+  int j = (int)(row0 - (unsigned)kb * (unsigned)ny) + 1;
+  size_t cb = cix(a, 1, j, kb % nz + 1, kb / nz);
+  float v[16];
   unsigned okb = 0;
 #pragma unroll
-  for (int it = 0; it < 16; ++it) {
-    const int x = xg * 64 + l;
-    const bool ok = G0 + it < ngroups && x < nfr;
-    v0[it] = a.c[ok ? cb + x : 0];
-    v1[it] = a.c[ok ? cb + x + d2 : 0];
+  for (int it = 0; it < 16; ++it) {  // all loads first (cells past the end read element 0)
+    const int x = xw * 64 + l;
+    const bool ok = w0 + it < nwords && x < nx;
+    v[it] = a.c[ok ? cb + x : 0];
     okb |= (unsigned)ok << it;
-    if (++xg == ngr) {  // next row
-      xg = 0;
-      if (++r == ry) {
-        r = 0;
-        if (++k == a.nz) {
-          k = 0;
-          ++b;
-        }
+    if (++xw == nw) {  // next row
+      xw = 0;
+      if (++j > ny) {
+        j = 1;
+        ++kb;
       }
-      cb = cix(a, 1, r + 1, k + 1, b);
+      cb = cix(a, 1, j, kb % nz + 1, kb / nz);
     }
   }
-  uint64_t mask = 0;
 #pragma unroll
   for (int it = 0; it < 16; ++it) {
-    const bool live = ((okb >> it) & 1) && fmaxf(fabsf(v0[it]), fabsf(v1[it])) > thr;
-    const uint64_t bal = __ballot(live);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) mask |= (uint64_t)(((bal >> (16 * q)) & 0xffffu) != 0) << (4 * it + q);
+    const unsigned long long m = __ballot(((okb >> it) & 1) && fabsf(v[it]) > thr);
+    if (l == 0 && w0 + it < nwords) bits[w0 + it] = m;
   }
-  if (t0 + l < ntile) flag[t0 + l] = (unsigned char)((mask >> l) & 1);
-  const int na = __popcll(mask);
-  if (na == 0) return;
-  unsigned base = 0;
-  if (l == 0) base = atomicAdd(count, (unsigned)na);
-  base = __shfl(base, 0, 64);
-  if ((mask >> l) & 1) {
-    const int rank = __popcll(mask & (((uint64_t)1 << l) - 1));
-    list[base + rank] = (int)(t0 + l);
+}
+
+// Activity, pass 2 (one axis). A face is live when one of its two cells is,
+// so an x tile T of row (j, k, b) (faces 16T+2 .. 16T+17) is active when one
+// of the cells 16T+1 .. 16T+17 of the row is live, a y tile T of face row
+// (f, k, b) when one of the cells 16T+1 .. 16T+16 of rows f-1 or f is (cells
+// past nx have no bit set). The pass writes one flag per tile and appends the
+// active tiles to the list. A workgroup takes kTilesPerThread x 256
+// consecutive tiles and makes one atomic for all of them (the order of the
+// list does not matter, every face is computed from its own stencil). Faces
+// of inactive tiles get no correction: the apply pass reads their flag, not
+// their dPhi. The host keeps the tile count below 2^31.
+constexpr int kTilesPerThread = 8;
+__device__ __forceinline__ bool tile_live(int axis, int nt, int nw, int ny,
+                                          const unsigned long long *__restrict__ bits, unsigned t) {
+  const unsigned row = t / (unsigned)nt;
+  const int T = (int)(t - row * (unsigned)nt);
+  const int w = T >> 2, o = 16 * (T & 3);
+  if (w >= nw) return false;
+  if (axis == 0) {  // bits 16T .. 16T+16 of cell row `row`
+    const unsigned long long *r = bits + (size_t)row * nw;
+    const unsigned long long lo = r[w] >> o;
+    const unsigned long long hi = (o == 48 && w + 1 < nw) ? (r[w + 1] & 1ull) << 16 : 0ull;
+    return ((lo | hi) & 0x1FFFFull) != 0;
+  }
+  // face row (f - 2) + (ny - 1) kb: cell rows f - 2 and f - 1 (0-based) of level kb
+  const unsigned kb = row / (unsigned)(ny - 1);
+  const unsigned f2 = row - kb * (unsigned)(ny - 1);
+  const unsigned long long *r = bits + ((size_t)f2 + (size_t)ny * kb) * nw;
+  return (((r[w] | r[w + nw]) >> o) & 0xFFFFull) != 0;
+}
+
+__global__ void __launch_bounds__(256) k_tile_flags(int axis, int nx, int ny, int nz, int nb,
+                                                    const unsigned long long *__restrict__ bits,
+                                                    unsigned char *__restrict__ flag, int *__restrict__ list,
+                                                    unsigned *__restrict__ count) {
+  __shared__ unsigned wcount[4], wbase[4];
+  int nt;
+  long rows;
+  face_space(axis, nx, ny, nz, nb, nt, rows);
+  const unsigned ntile = (unsigned)nt * (unsigned)rows;
+  const int nw = (nx + 63) / 64;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned tb = blockIdx.x * (256u * kTilesPerThread) + threadIdx.x;
+  unsigned long long m[kTilesPerThread];
+  unsigned livebits = 0, n = 0;
+#pragma unroll
+  for (int it = 0; it < kTilesPerThread; ++it) {
+    const unsigned t = tb + 256u * it;
+    bool live = false;
+    if (t < ntile) {
+      live = tile_live(axis, nt, nw, ny, bits, t);
+      flag[t] = (unsigned char)live;
+    }
+    m[it] = __ballot(live);
+    livebits |= (unsigned)live << it;
+    n += (unsigned)__popcll(m[it]);
+  }
+  // one atomic per workgroup; waves take consecutive ranges in wave order
+  if (l == 0) wcount[wv] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0;
+    for (int q = 0; q < 4; ++q) {
+      wbase[q] = tot;
+      tot += wcount[q];
+    }
+    const unsigned b0 = tot ? atomicAdd(count, tot) : 0u;
+    for (int q = 0; q < 4; ++q) wbase[q] += b0;
+  }
+  __syncthreads();
+  unsigned off = wbase[wv];
+  const unsigned long long below = (1ull << l) - 1;
+#pragma unroll
+  for (int it = 0; it < kTilesPerThread; ++it) {
+    if ((livebits >> it) & 1) list[off + __popcll(m[it] & below)] = (int)(tb + 256u * it);
+    off += (unsigned)__popcll(m[it]);
   }
 }
 
@@ -403,33 +449,57 @@ __device__ __forceinline__ SegFlags seg_flags(const unsigned char *__restrict__ 
 }
 
 // List of the 64-cell row segments with a face in an active tile (one thread
-// per segment, one atomic per wave). nsegs = nseg * ny * nz * nb < 2^31.
+// per segment, kTilesPerThread segments per thread, one atomic per
+// workgroup). nsegs = nseg * ny * nz * nb < 2^31.
 __global__ void __launch_bounds__(256) k_touched_segments(const unsigned char *__restrict__ flx,
                                                           const unsigned char *__restrict__ fly, int nx, int ny,
                                                           int nz, int nb, int nseg, int *__restrict__ list,
                                                           unsigned *__restrict__ count) {
+  __shared__ unsigned wcount[4], wbase[4];
   int ntx, nty;
   long rows_unused;
   face_space(0, nx, ny, nz, nb, ntx, rows_unused);
   face_space(1, nx, ny, nz, nb, nty, rows_unused);
   const unsigned nsegs = (unsigned)nseg * (unsigned)ny * (unsigned)nz * (unsigned)nb;
-  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
-  bool t = false;
-  if (g < nsegs) {
-    const unsigned row = g / (unsigned)nseg;  // (j - 1) + ny kb
-    const int sx = (int)(g - row * (unsigned)nseg);
-    const unsigned kb = row / (unsigned)ny;
-    const int j = (int)(row - kb * (unsigned)ny) + 1;
-    const SegFlags f = seg_flags(flx, fly, ntx, nty, ny, j, (int)kb, sx);
-    t = (f.wx | f.wp | f.wy0 | f.wy1) != 0u;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned gb = blockIdx.x * (256u * kTilesPerThread) + threadIdx.x;
+  unsigned long long m[kTilesPerThread];
+  unsigned tbits = 0, n = 0;
+#pragma unroll
+  for (int it = 0; it < kTilesPerThread; ++it) {
+    const unsigned g = gb + 256u * it;
+    bool t = false;
+    if (g < nsegs) {
+      const unsigned row = g / (unsigned)nseg;  // (j - 1) + ny kb
+      const int sx = (int)(g - row * (unsigned)nseg);
+      const unsigned kb = row / (unsigned)ny;
+      const int j = (int)(row - kb * (unsigned)ny) + 1;
+      const SegFlags f = seg_flags(flx, fly, ntx, nty, ny, j, (int)kb, sx);
+      t = (f.wx | f.wp | f.wy0 | f.wy1) != 0u;
+    }
+    m[it] = __ballot(t);
+    tbits |= (unsigned)t << it;
+    n += (unsigned)__popcll(m[it]);
   }
-  const uint64_t m = __ballot(t);
-  if (m == 0) return;  // uniform
-  const int l = threadIdx.x & 63;
-  unsigned base = 0;
-  if (l == 0) base = atomicAdd(count, (unsigned)__popcll(m));
-  base = __shfl(base, 0, 64);
-  if (t) list[base + __popcll(m & (((uint64_t)1 << l) - 1))] = (int)g;
+  if (l == 0) wcount[wv] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0;
+    for (int q = 0; q < 4; ++q) {
+      wbase[q] = tot;
+      tot += wcount[q];
+    }
+    const unsigned b0 = tot ? atomicAdd(count, tot) : 0u;
+    for (int q = 0; q < 4; ++q) wbase[q] += b0;
+  }
+  __syncthreads();
+  unsigned off = wbase[wv];
+  const unsigned long long below = (1ull << l) - 1;
+#pragma unroll
+  for (int it = 0; it < kTilesPerThread; ++it) {
+    if ((tbits >> it) & 1) list[off + __popcll(m[it] & below)] = (int)(gb + 256u * it);
+    off += (unsigned)__popcll(m[it]);
+  }
 }
 
 // Each wave takes listed segments (uniform), one cell per lane; a cell none
@@ -711,9 +781,13 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
     if (rel) a.smin_dev = c->lthr;
     a.gain = (float)c->learned_gain;
     const long nt = ntiles(c, axis);
-    const long chunks = (nt + 63) / 64;  // 64 tiles (16 groups) per wave, 4 waves per workgroup
-    hipLaunchKernelGGL(k_active_tiles, dim3((unsigned)((chunks + 3) / 4)), dim3(256), 0, c->stream, a, c->lflag[axis],
-                       c->lact[axis], c->lcount + axis);
+    if (axis == 0) {  // live cells of the state, for both axes: 16 words of 64 cells per wave
+      const long words = (long)c->ny * c->nz * c->nb * ((c->nx + 63) / 64);
+      hipLaunchKernelGGL(k_live_bits, dim3((unsigned)((words + 63) / 64)), dim3(256), 0, c->stream, a, c->lbits);
+    }
+    hipLaunchKernelGGL(k_tile_flags, dim3((unsigned)((nt + 256 * kTilesPerThread - 1) / (256 * kTilesPerThread))),
+                       dim3(256), 0, c->stream, axis, c->nx, c->ny,
+                       c->nz, c->nb, c->lbits, c->lflag[axis], c->lact[axis], c->lcount + axis);
     hipLaunchKernelGGL(k_correct_faces, dim3(grid_for(nt)), dim3(256), 0, c->stream, a, c->learned->packed,
                        c->dphi[axis], c->lact[axis], c->lcount + axis);
     const hipError_t e = hipGetLastError();
@@ -726,7 +800,8 @@ int learned_post_step(fall3d_ads_ctx *c) {
   const int nseg = (c->nx + 63) / 64;
   const long nsegs = (long)c->ny * c->nz * c->nb * nseg;
   const float lim = (float)c->learned_flux_limit;
-  hipLaunchKernelGGL(k_touched_segments, dim3((unsigned)((nsegs + 255) / 256)), dim3(256), 0, c->stream, c->lflag[0],
+  hipLaunchKernelGGL(k_touched_segments, dim3((unsigned)((nsegs + 256 * kTilesPerThread - 1) / (256 * kTilesPerThread))),
+                     dim3(256), 0, c->stream, c->lflag[0],
                      c->lflag[1], c->nx, c->ny, c->nz, c->nb, nseg, c->lseg, c->lcount + 2);
   const unsigned blocks = (unsigned)std::min<long>((nsegs + 3) / 4, 8192);
   hipLaunchKernelGGL(k_apply_corrections, dim3(blocks), dim3(256), 0, c->stream, c->chi, c->dphi[0], c->dphi[1],
@@ -754,6 +829,8 @@ void learned_release(fall3d_ads_ctx *c) {
   }
   dfree(c->lseg);
   c->lseg = nullptr;
+  dfree(c->lbits);
+  c->lbits = nullptr;
   dfree(c->lcount);
   c->lcount = nullptr;
   c->learned = nullptr;
@@ -986,6 +1063,7 @@ int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, doub
       return fail(c, FALL3D_ADS_EINVAL, "domain too large for the learned correction's tile list");
     if (!c->lcount) HIPCHK(c, dalloc(&c->lcount, 3));
     if (!c->lseg) HIPCHK(c, dalloc(&c->lseg, (size_t)c->ny * c->nz * c->nb * ((c->nx + 63) / 64)));
+    if (!c->lbits) HIPCHK(c, dalloc(&c->lbits, (size_t)c->ny * c->nz * c->nb * ((c->nx + 63) / 64)));
     for (int a = 0; a < 2; ++a) {
       if (!c->dphi[a]) HIPCHK(c, dalloc(&c->dphi[a], nface(c, a)));
       if (!c->lact[a]) HIPCHK(c, dalloc(&c->lact[a], (size_t)ntiles(c, a)));
