@@ -6,7 +6,8 @@ builds both from one seed); fall3d_ads_learned_collect turns one coarse step
 of the coarse-grained fine state into (face features -> flux correction)
 rows on the device, and the network trains there (fall3d_mlp_train). Plain
 KT on the coarse grid is the fallback and the baseline the correction is
-judged against; the judge is the fine solution averaged onto the coarse grid.
+measured against; the reference solution is the fine run averaged onto the
+coarse grid.
 """
 from __future__ import annotations
 

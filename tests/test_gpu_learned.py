@@ -1,7 +1,8 @@
 """The learned flux correction end to end on a small grid: the solver
 generates (coarse face -> flux correction) rows from fine and coarse runs,
 the network trains on the GPU, and the corrected coarse run is compared with
-plain coarse KT, both judged by the fine run averaged onto the coarse grid.
+plain coarse KT, both measured against the fine run averaged onto the
+coarse grid.
 """
 import numpy as np
 import pytest

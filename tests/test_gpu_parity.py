@@ -3,7 +3,7 @@
 Tolerances: the device computes reconstruction, fluxes and stage updates in
 FP32 (FP64 only for the boundary/deposit accumulators, CFL and budget), so
 concentrations and deposits are compared with relative L2 <= 1e-5 (the
-north-star bar) and max-norm <= 1e-4 of the field's maximum; FP64-only
+required FP32 accuracy) and max-norm <= 1e-4 of the field's maximum; FP64-only
 paths (ADS_solve_1D) with 1e-12. Mass conservation of the compensated
 state (fp32x2) is checked to 1e-12 relative.
 """
