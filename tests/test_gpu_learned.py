@@ -116,13 +116,15 @@ def _corrections_cpu(prob, c0, dt, net, smin, gain):
     return out, stats
 
 
-def test_learned_step_applies_the_network_corrections():
+@pytest.mark.parametrize("threshold", ["absolute", "relative"])
+def test_learned_step_applies_the_network_corrections(threshold):
     """One learned step equals the plain step minus the divergence of the
     face corrections restated on the CPU (features from the oracle, y from
     the network's own forward pass, which is pinned against the MLP oracle
     separately). A ragged grid (tiles of 16 faces with partial last tiles),
-    a threshold that leaves some tiles inactive, and an output layer scaled
-    so that part of the faces reach the positivity clamp."""
+    a threshold that leaves some tiles inactive (absolute, or relative to the
+    field's largest |c| and computed on the device), and an output layer
+    scaled so that part of the faces reach the positivity clamp."""
     from fall3d_amd.problem import config1_wind_problem
 
     p = config1_wind_problem(37, 29, 6, 2, seed=5, dx=5000.0, dz=1000.0, nsources=1, puff=True)
@@ -134,11 +136,16 @@ def test_learned_step_applies_the_network_corrections():
     params = ads.Network.initial_params(11)
     params[-65:] *= 300.0  # w3, b3: outputs large enough to meet the clamp
     net = ads.Network(params=params)
-    smin, gain = 1e-3 * float(np.abs(c0).max()), 0.5
+    gain = 0.5
+    if threshold == "absolute":
+        smin = dev_smin = 1e-3 * float(np.abs(c0).max())
+    else:  # the device's threshold: fp32(1e-3) * max |c| over the whole array, in FP32
+        dev_smin = -1e-3
+        smin = float(np.float32(1e-3) * np.float32(np.abs(c0).max()))
     res = []
     for learned in (False, True):
         g.set_state(c0, a0)
-        g.set_learned(net if learned else None, smin=smin, gain=gain)
+        g.set_learned(net if learned else None, smin=dev_smin, gain=gain)
         g.step(dt, 0)
         res.append(g.get_state()[0])
     g.release()
