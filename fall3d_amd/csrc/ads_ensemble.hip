@@ -260,6 +260,8 @@ void default_tiling(fall3d_ads_ensemble *e) {
       e->x_cpt = kSweepCpt[i];
       break;
     }
+  // (the autotuner may also pick shapes whose widest lines take two waves:
+  // their workgroups keep the 256 threads every member's launch must share)
   e->bpg[1] = e->bpg[2] = std::min(e->nb, 8);
 }
 
@@ -402,7 +404,7 @@ int fall3d_ads_ensemble_set_tiling(fall3d_ads_ensemble *e, int x_cells_per_lane,
     if (!sweep_cpt_supported(x_cells_per_lane))
       return efail(e, FALL3D_ADS_EINVAL, "unsupported cells per lane %d", x_cells_per_lane);
     for (auto *c : e->mem)
-      if (c->nx / x_cells_per_lane + 1 > 64)
+      if (c->nx / x_cells_per_lane + 1 > 128)  // at most two waves per line: 256-thread workgroups
         return efail(e, FALL3D_ADS_EINVAL, "cells per lane %d too small for nx = %d", x_cells_per_lane, c->nx);
     e->x_cpt = x_cells_per_lane;
   }
@@ -520,7 +522,7 @@ int fall3d_ads_ensemble_autotune(fall3d_ads_ensemble *e, const double *dt, const
   }
   std::vector<int> xc, yz;
   for (int i = 0; i < kSweepCptCount; ++i)
-    if (nxmax / kSweepCpt[i] + 1 <= 64) xc.push_back(kSweepCpt[i]);
+    if (nxmax / kSweepCpt[i] + 1 <= 128) xc.push_back(kSweepCpt[i]);  // up to two waves per line
   for (int b = 1; b <= (comp ? 1 : 2); ++b)
     for (int g = b; g <= std::min(e->nb, 8); ++g) {
       const int groups = (e->nb + g - 1) / g;

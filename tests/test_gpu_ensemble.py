@@ -69,6 +69,27 @@ def test_ensemble_members_bit_identical_to_single_runs(state, tm):
     ens.release()
 
 
+def test_ensemble_two_wave_lines_bit_identical():
+    """Two cells per lane: the wider members' x lines take two waves of a
+    workgroup, the narrow one's a single wave, in the same batched launch;
+    each member bit-identical to its single run with that tile shape."""
+    specs = ensemble_members(3, nb=2, seed=8, nxy=(40, 200), nzr=(8, 12))  # nx 155, 92, 77
+    ens = ads.Ensemble([(s.nx, s.ny, s.nz) for s in specs], specs[0].nb, "minmod", "rk", 8, "fp32")
+    dts = _fill(ens, specs)
+    ens.set_tiling(2, 0, 0)
+    tiling = ens.tiling()
+    assert tiling[0] == 2 and any(sp.nx // 2 + 1 > 64 for sp in specs) and any(sp.nx // 2 + 1 <= 64 for sp in specs)
+    nsteps = 5
+    ens.run(dts, nsteps, 0)
+    ens.synchronize()
+    for m, sp in enumerate(specs):
+        c_e, a_e = ens.members[m].get_state()
+        c_s, a_s, _ = _alone(sp, tiling, "fp32", "rk", dts[m], nsteps)
+        assert np.array_equal(c_e, c_s), m
+        assert np.array_equal(a_e, a_s), m
+    ens.release()
+
+
 def test_ensemble_matches_oracle():
     specs = _members(3, nb=2, seed=5)
     ens = ads.Ensemble([(s.nx, s.ny, s.nz) for s in specs], specs[0].nb, "minmod", "rk", 8, "fp32")
