@@ -232,14 +232,14 @@ def test_deterministic_and_tiling_invariant():
         np.testing.assert_allclose(led, outs[0][1], rtol=1e-12, atol=1e-12 * np.abs(outs[0][1]).max())
 
 
-@pytest.mark.parametrize("state", ["fp32", "fp32x2"])
-def test_multiwave_x_lines_bit_identical(state):
+@pytest.mark.parametrize("state,tm", [("fp32", "rk"), ("fp32x2", "rk"), ("fp32", "euler")])
+def test_multiwave_x_lines_bit_identical(state, tm):
     """Lines held by 2 or 3 waves (edge cells traded through LDS) give the
     same bits as lines held by one wave."""
     p = operational_problem(200, 9, 6, 2, seed=8)
     outs = []
     for cpt in (4, 2, 3):  # 51 segments: one wave; 101 and 67: two waves
-        g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, state)
+        g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", tm, 8, state)
         g.set_problem(p)
         g.set_tiling(0, cells_per_thread=cpt)
         dt = g.time_step(0.0, 2, 0.9)
