@@ -16,6 +16,12 @@
 //   dPhi = sgn_n * S * dxp * 0.01 y   (mass per unit face area over the step),
 // so the correction scales with the field (the transport is linear in c).
 //
+// Only faces in active tiles are corrected: a tile is 16 consecutive faces
+// along x, active when one of its faces has a cell above the threshold. One
+// step runs: live-cell bits (one read of c), tile flags and a list of the
+// active tiles per axis, the network on the listed tiles, a list of the row
+// segments next to an active tile, and the update of those segments' cells.
+//
 // Training pairs come from the solver itself: a fine run and a coarse run
 // from the coarse-grained fine state, both with the sweeps' face-flux output
 // on; the target of a coarse face is (fine flux averaged over its two fine
