@@ -350,11 +350,14 @@ void ora_solve_z(const int *cfg, const double *const *grid, double dt,
   sweep(2, cfg, grid, dt, c, k3, w1, w2, stime, vs, acum, Df, Uf, ledger);
 }
 
-/* Sources/mod_F3D.F90:1456-1473: c += dt*s/vol, vol = dX1_p dX2_p dX3_p
- * (Hm = 1 for the flat Cartesian grids this restatement supports). Returns
- * the injected mass sum(dt*s) in the budget's measure. */
-double ora_add_source(const int *cfg, const double *const *grid, double dt,
-                      double *c, const double *s) {
+/* Sources/mod_F3D.F90:1456-1473: c += dt*s/vol with
+ * vol = dX1_p(i) dX2_p(j) dX3_p(k) Hm1_p(j) Hm2_p(j) Hm3_p(i,j). The map
+ * factors are optional: hm1(1:ny), hm2(-1:ny+2), hm3(1:nx,1:ny), NULL = 1
+ * (flat Cartesian grid, Sources/mod_Grid.F90:403,463,603). Returns the
+ * injected mass sum(dt*s/(Hm1 Hm2 Hm3)) in the dX_p measure. */
+double ora_add_source_metric(const int *cfg, const double *const *grid, double dt,
+                             double *c, const double *s, const double *hm1,
+                             const double *hm2, const double *hm3) {
   const int nx = cfg[CFG_NX], ny = cfg[CFG_NY], nz = cfg[CFG_NZ], nb = cfg[CFG_NB];
   const double *dx1p = grid[0] + 1, *dx2p = grid[2] + 1, *dx3p = grid[4] + 1;
   double injected = 0.0;
@@ -362,18 +365,26 @@ double ora_add_source(const int *cfg, const double *const *grid, double dt,
     double dZ = dx3p[k];
     for (int j = 1; j <= ny; ++j) {
       double dY = dx2p[j];
+      double Hm1 = hm1 ? hm1[j - 1] : 1.0;
+      double Hm2 = hm2 ? hm2[j + 1] : 1.0;
       for (int i = 1; i <= nx; ++i) {
         double dX = dx1p[i];
-        double vol = dX * dY * dZ * (1.0 * 1.0 * 1.0);
+        double Hm3 = hm3 ? hm3[(size_t)(i - 1) + (size_t)nx * (size_t)(j - 1)] : 1.0;
+        double vol = dX * dY * dZ * (Hm1 * Hm2 * Hm3);
         for (int b = 0; b < nb; ++b) {
           size_t si = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)(k - 1) + (size_t)nz * b));
           c[CIDX(i, j, k, b)] = c[CIDX(i, j, k, b)] + dt * s[si] / vol;
-          injected += dt * s[si];
+          injected += dt * s[si] / (Hm1 * Hm2 * Hm3);
         }
       }
     }
   }
   return injected;
+}
+
+double ora_add_source(const int *cfg, const double *const *grid, double dt,
+                      double *c, const double *s) {
+  return ora_add_source_metric(cfg, grid, dt, c, s, NULL, NULL, NULL);
 }
 
 /* Sources/mod_Grid.F90:1026-1141, criterion 1 (per direction) or 2 (all

@@ -17,6 +17,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "libads_oracle.so")
+# A different build of the oracle (the planted-mistake builds of
+# tests/test_oracle_mutants.py) can be loaded instead through this variable.
+_LIB_OVERRIDE = os.environ.get("FALL3D_ORACLE_LIB")
 _lib = None
 
 LIMITERS = {"minmod": 1, "superbee": 2, "ospre": 3}
@@ -35,9 +38,10 @@ def build() -> str:
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        path = _LIB_OVERRIDE or _LIB_PATH
+        if not _LIB_OVERRIDE and not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         for name in ("ora_solve_x", "ora_solve_y"):
             f = getattr(L, name)
             f.argtypes = [_ip, ctypes.c_void_p, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p]
@@ -47,6 +51,9 @@ def lib():
         L.ora_solve_z.restype = None
         L.ora_add_source.argtypes = [_ip, ctypes.c_void_p, ctypes.c_double, _dp, _dp]
         L.ora_add_source.restype = ctypes.c_double
+        L.ora_add_source_metric.argtypes = [_ip, ctypes.c_void_p, ctypes.c_double, _dp, _dp, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+        L.ora_add_source_metric.restype = ctypes.c_double
         L.ora_time_step.argtypes = [_ip, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
                                     ctypes.c_double]
         L.ora_time_step.restype = ctypes.c_double
@@ -112,8 +119,14 @@ class Oracle:
         self.flux[4], self.flux[5] = D.value, U.value
         self.ledger[4:6] += led
 
-    def add_source(self, dt, c, s):
-        return lib().ora_add_source(self._cfg, self._grid, dt, _p(c), _p(s))
+    def add_source(self, dt, c, s, hm1=None, hm2=None, hm3=None):
+        """c += dt s / (dX1_p dX2_p dX3_p Hm1_p Hm2_p Hm3_p); map factors
+        hm1 (ny), hm2 (ny+4, 2 halo), hm3 (nx, ny) or None (= 1)."""
+        if hm1 is None and hm2 is None and hm3 is None:
+            return lib().ora_add_source(self._cfg, self._grid, dt, _p(c), _p(s))
+        hs = [None if h is None else np.asfortranarray(h, dtype=np.float64) for h in (hm1, hm2, hm3)]
+        return lib().ora_add_source_metric(self._cfg, self._grid, dt, _p(c), _p(s),
+                                           *[None if h is None else h.ctypes.data for h in hs])
 
     def time_step(self, u, v, w, vs, criterion=2, cfl=0.9):
         p = self.prob
