@@ -44,7 +44,7 @@ EXPORTS = (
     "fall3d_ads_initialize", "fall3d_ads_set_grid", "fall3d_ads_get_vcomps",
     "fall3d_ads_update_horizontal_vcomps", "fall3d_ads_solve_along_x", "fall3d_ads_solve_along_y",
     "fall3d_ads_solve_along_z", "fall3d_ads_solve_1d", "fall3d_ads_release", "fall3d_ads_last_error",
-    "fall3d_ads_set_meteo", "fall3d_ads_set_source", "fall3d_ads_set_state", "fall3d_ads_get_state",
+    "fall3d_ads_set_meteo", "fall3d_ads_set_source", "fall3d_ads_set_source_metric", "fall3d_ads_set_state", "fall3d_ads_get_state",
     "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
@@ -95,6 +95,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_last_error": ([_vp], ctypes.c_char_p),
         "fall3d_ads_set_meteo": ([_vp] * 11, _int),
         "fall3d_ads_set_source": ([_vp, _vp], _int),
+        "fall3d_ads_set_source_metric": ([_vp, _vp, _vp, _vp, _vp], _int),
         "fall3d_ads_set_state": ([_vp, _vp, _vp], _int),
         "fall3d_ads_get_state": ([_vp, _vp, _vp], _int),
         "fall3d_ads_get_time_step": ([_vp, _dbl, _int, _dbl, _dp], _int),
@@ -202,7 +203,10 @@ class ADS:
             a = a.astype(self.dtype)
         if a.size != int(np.prod(shape)):
             raise ValueError(f"{name}: expected {shape} elements, got shape {a.shape}")
-        if not (a.flags.f_contiguous or a.ndim == 1):
+        # a strided view (also a 1-D one such as x[::2]) would hand the
+        # library the wrong elements: read-only inputs are copied, writable
+        # arguments must already be contiguous
+        if not a.flags.f_contiguous:
             if writable:
                 raise ValueError(f"{name}: must be Fortran-contiguous")
             a = np.asfortranarray(a)
@@ -348,9 +352,18 @@ class ADS:
         self.set_source(prob.s)
         self.set_state(prob.c0)
 
-    def set_source(self, s):
+    def set_source(self, s, hm1=None, hm2=None, hm3=None):
+        """Source rate s(1:nx,1:ny,1:nz,nb); optional map factors Hm1_p(1:ny),
+        Hm2_p(-1:ny+2), Hm3_p(1:nx,1:ny) (None = 1, the flat Cartesian grid)."""
         s = self._arr(s, (self.nx, self.ny, self.nz, self.nb), "s") if s is not None else None
-        self._chk(load_library().fall3d_ads_set_source(self._h, self._p(s)), "set_source")
+        if hm1 is None and hm2 is None and hm3 is None:
+            self._chk(load_library().fall3d_ads_set_source(self._h, self._p(s)), "set_source")
+            return
+        h1 = self._arr(hm1, (self.ny,), "Hm1_p")
+        h2 = self._arr(hm2, (self.ny + 4,), "Hm2_p")
+        h3 = self._arr(hm3, (self.nx, self.ny), "Hm3_p")
+        self._chk(load_library().fall3d_ads_set_source_metric(self._h, self._p(s), self._p(h1), self._p(h2),
+                                                              self._p(h3)), "set_source_metric")
 
     def set_state(self, c, acum=None):
         """Upload c (None: zero field) and acum (None: zero)."""

@@ -497,6 +497,21 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     return fail(c, FALL3D_ADS_EINVAL, "invalid time_marching %d", time_marching);
   if (real_kind != 4 && real_kind != 8) return fail(c, FALL3D_ADS_EINVAL, "invalid real kind %d", real_kind);
   if (state_mode != 0 && state_mode != 1) return fail(c, FALL3D_ADS_EINVAL, "invalid state mode %d", state_mode);
+  {
+    // The sweeps address each bin's concentration, and each wind, diffusion
+    // and settling-velocity array, through a buffer descriptor whose record
+    // count is a 32-bit byte size and drop masked stores with an offset of
+    // 2^31: every such array must stay below 2^31 bytes.
+    const size_t lim = (size_t)1 << 31;
+    const size_t per_bin_c = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * sizeof(float);
+    const size_t faces = std::max({(size_t)(nx + 4) * ny * nz, (size_t)nx * (ny + 4) * nz, (size_t)nx * ny * (nz + 4)}) *
+                         sizeof(float);
+    if (per_bin_c >= lim || faces >= lim)
+      return fail(c, FALL3D_ADS_EINVAL,
+                  "grid %d x %d x %d too large: one bin of the concentration (with ghosts) or one wind/diffusion "
+                  "field must stay below 2 GiB",
+                  nx, ny, nz);
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(c, FALL3D_ADS_EHIP, "no HIP device available");
@@ -616,6 +631,9 @@ int fall3d_ads_set_grid(fall3d_ads_ctx *c, const void *dX1_p, const void *dX1_b,
   }
   c->grid_ok = true;
   c->dtib_ok = false;
+  // the per-axis tables (dt/dxb, iteration factors) and every captured
+  // graph or batched-launch table built from the old metrics are stale
+  c->version++;
   return FALL3D_ADS_OK;
 }
 
@@ -764,7 +782,8 @@ int fall3d_ads_set_meteo(fall3d_ads_ctx *c, const void *u1, const void *u2, cons
   return FALL3D_ADS_OK;
 }
 
-int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
+int fall3d_ads_set_source_metric(fall3d_ads_ctx *c, const void *s, const void *Hm1_p, const void *Hm2_p,
+                                 const void *Hm3_p) {
   if (!c) return FALL3D_ADS_EINVAL;
   int rc;
   if ((rc = need_grid(c))) return rc;
@@ -779,9 +798,19 @@ int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
   c->version++;
   if (!s) return FALL3D_ADS_OK;
   const int nx = c->nx, ny = c->ny, nz = c->nz;
+  // map factors Hm1_p(1:ny), Hm2_p(-1:ny+2), Hm3_p(1:nx,1:ny); NULL = 1
+  auto hm1 = [&](int j) { return Hm1_p ? host_val(Hm1_p, c->kind, (size_t)(j - 1)) : 1.0; };
+  auto hm2 = [&](int j) { return Hm2_p ? host_val(Hm2_p, c->kind, (size_t)(j + 1)) : 1.0; };
+  auto hm3 = [&](int i, int j) {
+    return Hm3_p ? host_val(Hm3_p, c->kind, (size_t)(i - 1) + (size_t)nx * (size_t)(j - 1)) : 1.0;
+  };
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      if (!(hm1(j) * hm2(j) * hm3(i, j) > 0.0))
+        return fail(c, FALL3D_ADS_EINVAL, "non-positive map factor product at cell (%d,%d)", i, j);
   std::vector<size_t> idx;
   std::vector<double> rate, meas;
-  // Sources/mod_F3D.F90:1456-1473: c += dt*s/vol, vol = dX1_p dX2_p dX3_p (Hm = 1)
+  // Sources/mod_F3D.F90:1456-1473: c += dt*s/vol, vol = dX1_p dX2_p dX3_p Hm1 Hm2 Hm3
   for (int b = 0; b < c->nb; ++b)
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
@@ -789,7 +818,8 @@ int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
           const size_t si = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)(k - 1) + (size_t)nz * b));
           const double v = host_val(s, c->kind, si);
           if (v == 0.0) continue;
-          const double vol = c->hdxp64[0][i + 1] * c->hdxp64[1][j + 1] * c->hdxp64[2][k + 1];
+          const double vol = c->hdxp64[0][i + 1] * c->hdxp64[1][j + 1] * c->hdxp64[2][k + 1] *
+                             (hm1(j) * hm2(j) * hm3(i, j));
           idx.push_back((size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b)));
           rate.push_back(v / vol);
           meas.push_back(c->hdxb64[0][i] * c->hdxb64[1][j] * c->hdxb64[2][k]);
@@ -806,6 +836,10 @@ int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
   HIPCHK(c, hipMemcpy(c->src_rate, rate.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->src_meas, meas.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
   return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_set_source(fall3d_ads_ctx *c, const void *s) {
+  return fall3d_ads_set_source_metric(c, s, nullptr, nullptr, nullptr);
 }
 
 int fall3d_ads_set_state(fall3d_ads_ctx *c, const void *cst, const void *acum) {

@@ -74,8 +74,8 @@ class Problem:
         return [(int(i) + 1, int(j) + 1, int(k) + 1, int(b), float(self.s[i, j, k, b])) for i, j, k, b in idx]
 
 
-def _F(shape):
-    return np.zeros(shape, dtype=np.float64, order="F")
+def _F(shape, dtype=np.float64):
+    return np.zeros(shape, dtype=dtype, order="F")
 
 
 def uniform_grid(nx, ny, nz, dx, dy, dz):
@@ -113,11 +113,11 @@ def sigma_grid(nx, ny, nz, dx, dy, sigma, x3max):
     return g
 
 
-def constant_diffusion(nx, ny, nz, kh, kv):
+def constant_diffusion(nx, ny, nz, kh, kv, dtype=np.float64):
     """Constant K fills every element, halos included (Sources/mod_Phys.F90:479-480,590)."""
-    k1 = _F((nx + 4, ny, nz)); k1[...] = kh
-    k2 = _F((ny + 4, nx, nz)); k2[...] = kh
-    k3 = _F((nz + 4, nx, ny)); k3[...] = kv
+    k1 = _F((nx + 4, ny, nz), dtype); k1[...] = kh
+    k2 = _F((ny + 4, nx, nz), dtype); k2[...] = kh
+    k3 = _F((nz + 4, nx, ny), dtype); k3[...] = kv
     return k1, k2, k3
 
 
@@ -274,7 +274,7 @@ def operational_problem(nx=360, ny=240, nz=60, nb=10, dx=5000.0, dy=5000.0, dz=3
 
 def config1_wind_problem(nx, ny, nz, nb, dx=5000.0, dz=500.0, seed=0, coarsen=1, vmin=0.05, vmax=2.0,
                          kh=100.0, kv=10.0, nsources=1, column_top=15000.0, mass_rate=1.0e6, puff=True,
-                         sigma_cells=4.0, peak=1.0, zc=8000.0) -> Problem:
+                         sigma_cells=4.0, peak=1.0, zc=8000.0, dtype=np.float64) -> Problem:
     """The puff problem's wind generator over open boundaries, with seeded
     direction and sources; fine/coarse pairs for the learned correction.
 
@@ -301,21 +301,21 @@ def config1_wind_problem(nx, ny, nz, nb, dx=5000.0, dz=500.0, seed=0, coarsen=1,
     zk = (np.arange(1, nz + 1) - 0.5) * dz
     sp = 10.0 + 5.0 * np.sin(2.0 * math.pi * zk / H)
     ct, st = math.cos(theta), math.sin(theta)
-    u = _F((cx + 3, cy, nz))
+    u = _F((cx + 3, cy, nz), dtype)
     u[...] = (sp * ct - 3.0 * st)[None, None, :]
-    v = _F((cy + 3, cx, nz))
+    v = _F((cy + 3, cx, nz), dtype)
     v[...] = (sp * st + 3.0 * ct)[None, None, :]
-    w = _F((nz + 3, cx, cy))
-    k1, k2, k3 = constant_diffusion(cx, cy, nz, kh, kv)
+    w = _F((nz + 3, cx, cy), dtype)
+    k1, k2, k3 = constant_diffusion(cx, cy, nz, kh, kv, dtype)
     vset = vmin * (vmax / vmin) ** (np.arange(nb) / max(nb - 1, 1)) if nb > 1 else np.array([0.5])
-    vs = _F((cx, cy, nz + 3, nb))
+    vs = _F((cx, cy, nz + 3, nb), dtype)
     vs[...] = vset[None, None, None, :]
-    s = _F((cx, cy, nz, nb))
+    s = _F((cx, cy, nz, nb), dtype)
     for (i, j) in fx:
         ks = [k for k in range(1, nz + 1) if zk[k - 1] <= column_top]
         ic, jc = (i - 1) // coarsen + 1, (j - 1) // coarsen + 1
         s[ic - 1, jc - 1, [k - 1 for k in ks], :] += mass_rate / (nb * len(ks))
-    c = _F((cx + 4, cy + 4, nz + 4, nb))
+    c = _F((cx + 4, cy + 4, nz + 4, nb), dtype)
     if puff:
         xf = (np.arange(1, nx + 1) - 0.5) * dx
         sx = sigma_cells * dx

@@ -87,6 +87,9 @@ typedef struct fall3d_ads_ctx fall3d_ads_ctx;
  * (FALL3D_ADS_STATE_*). */
 int fall3d_ads_initialize(fall3d_ads_ctx **ctx, int nx, int ny, int nz, int nbins, int limiter,
                           int time_marching, int real_kind, int state_mode);
+/* Size limit: one bin of c with its ghosts, (nx+4)(ny+4)(nz+4) floats, and
+ * each wind/diffusion field must stay below 2 GiB (the sweeps address them
+ * with 32-bit buffer offsets); larger grids are rejected with EINVAL. */
 
 /* The ARAKAWA_C_GRID spacing arrays ADS reads (Sources/mod_KindType.F90:486-491):
  * dX1_p(-1:nx+2), dX1_b(0:nx+2), ... Must be called before any solve. */
@@ -148,9 +151,17 @@ int fall3d_ads_set_meteo(fall3d_ads_ctx *ctx, const void *u1, const void *u2, co
                          const void *w1, const void *w2, const void *k1, const void *k2, const void *k3,
                          const void *vs);
 
-/* Dense source rate s(1:nx,1:ny,1:nz,nbins) (MY_TRA%my_s); stored sparsely
- * on the device (only nonzero entries are injected). NULL clears it. */
+/* Dense source rate s(1:nx,1:ny,1:nz,nbins) (MY_TRA%my_s), injected before
+ * the sweeps as c += dt s / (dX1_p dX2_p dX3_p Hm1 Hm2 Hm3)
+ * (Sources/mod_F3D.F90:1456-1473); stored sparsely on the device (only
+ * nonzero entries are injected). NULL clears it. set_source takes all map
+ * factors as 1: that is FALL3D's flat Cartesian grid (Sources/mod_Grid.F90:
+ * 403,463,603). On spherical or terrain-following grids pass the grid's
+ * MY_GRID%Hm1_p(1:ny), Hm2_p(-1:ny+2) and Hm3_p(1:nx,1:ny) (real_kind, any
+ * may be NULL = 1) to set_source_metric. */
 int fall3d_ads_set_source(fall3d_ads_ctx *ctx, const void *s);
+int fall3d_ads_set_source_metric(fall3d_ads_ctx *ctx, const void *s, const void *Hm1_p, const void *Hm2_p,
+                                 const void *Hm3_p);
 
 /* Concentration (with ghosts) and ground accumulation; NULL sets zero. */
 int fall3d_ads_set_state(fall3d_ads_ctx *ctx, const void *c, const void *acum);

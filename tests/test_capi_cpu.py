@@ -49,3 +49,27 @@ def test_fortran_shim_declares_every_module_ads_procedure():
                  "ADS_solve_along_x", "ADS_solve_along_y", "ADS_solve_along_z", "ADS_solve_1D"):
         assert re.search(r"PUBLIC\s*::\s*" + name + r"\b", src), name
         assert re.search(r"subroutine\s+" + name + r"\s*\(", src, re.I), name
+
+
+def test_strided_arrays_are_copied_or_refused():
+    """A strided view (1-D included) never reaches the library as is: read-only
+    inputs are copied to contiguous memory, writable arguments are refused."""
+    import numpy as np
+
+    obj = ads.ADS.__new__(ads.ADS)
+    obj.dtype = np.float64
+    base = np.arange(20, dtype=np.float64)
+    view = base[::2]
+    out = obj._arr(view, (10,), "x")
+    assert out.flags.f_contiguous and np.array_equal(out, view)
+    with pytest.raises(ValueError):
+        obj._arr(view, (10,), "x", writable=True)
+    c2 = np.zeros((6, 5), order="C")
+    assert obj._arr(c2, (6, 5), "y").flags.f_contiguous
+
+
+def test_initialize_rejects_grids_past_the_32bit_buffer_limit():
+    if not os.path.exists(ads.LIB_PATH):
+        pytest.skip("library not built")
+    with pytest.raises(ads.Fall3dAdsError, match="2 GiB"):
+        ads.ADS(4096, 4096, 40, 1)  # one bin of c: 4100*4100*44*4 B > 2 GiB

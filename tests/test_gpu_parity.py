@@ -45,15 +45,36 @@ def _flux_scale(p, dt):
     return vmax * np.abs(p.c0).max() * area * dt * nl
 
 
+def _stretch_all_axes(p, seed):
+    """Stretched metrics on every axis: a sigma grid in z (random_problem's
+    stretched option, Sources/mod_Grid.F90:541-562) and independent random
+    dX_p / dX_b arrays in x and y, so no two neighbouring cells share a
+    spacing and dxp differs from dxb everywhere."""
+    rng = np.random.default_rng(1000 + seed)
+    g = p.grid
+    for ax, n, d in ((1, p.nx, 1000.0), (2, p.ny, 1200.0)):
+        g[f"dX{ax}_p"] = rng.uniform(0.6, 1.4, n + 4) * d
+        g[f"dX{ax}_b"] = rng.uniform(0.6, 1.4, n + 3) * d
+    return p
+
+
+def _problem(seed, grid, *shape, **kw):
+    if grid == "uniform":
+        return random_problem(*shape, seed=seed, **kw)
+    return _stretch_all_axes(random_problem(*shape, seed=seed, stretched=True, **kw), seed)
+
+
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("grid", ["uniform", "stretched"])
 @pytest.mark.parametrize("state", ["fp32", "fp32x2"])
 @pytest.mark.parametrize("tm", [1, 2])
 @pytest.mark.parametrize("lim", [1, 2, 3])
-def test_dropin_sweeps_match_oracle(lim, tm, state):
+def test_dropin_sweeps_match_oracle(lim, tm, state, grid):
     """ADS_solve_along_x/y/z through the drop-in entry points, random problem
-    (every face-velocity sign, ragged segment sizes, plateaus and zeros)."""
-    print(f"case lim={lim} tm={tm} state={state}", flush=True)
-    p = random_problem(13, 11, 9, 3, seed=7 * lim + tm)
+    (every face-velocity sign, ragged segment sizes, plateaus and zeros), on
+    uniform and on stretched metrics of every axis."""
+    print(f"case lim={lim} tm={tm} state={state} grid={grid}", flush=True)
+    p = _problem(7 * lim + tm, grid, 13, 11, 9, 3)
     o = Oracle(p, lim, tm)
     g = ads.ADS(p.nx, p.ny, p.nz, p.nb, lim, tm, 8, state, grid=p.grid)
     u, v, w = g.get_vcomps()
@@ -106,11 +127,14 @@ def test_update_horizontal_vcomps():
 
 
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("grid", ["uniform", "stretched"])
 @pytest.mark.parametrize("state", ["fp32", "fp32x2"])
-def test_resident_steps_match_oracle(state):
+def test_resident_steps_match_oracle(state, grid):
     """fall3d_ads_step = F3D_time_step (source, x/y order by parity, z) with
-    time-interpolated winds (stime != 0), several steps."""
-    p = random_problem(17, 12, 10, 2, seed=31, umax_scale=0.5)
+    time-interpolated winds (stime != 0), several steps, on uniform and
+    stretched metrics."""
+    print(f"case state={state} grid={grid}", flush=True)
+    p = _problem(31, grid, 17, 12, 10, 2, umax_scale=0.5)
     o = Oracle(p, 1, 2)
     g = ads.ADS(p.nx, p.ny, p.nz, p.nb, 1, 2, 8, state)
     g.set_problem(p)
@@ -361,4 +385,29 @@ def test_x_segmentation_edge_cases(nx, cpt):
     check_field(cg, cr, "x sweep c")
     scale = _flux_scale(p, dt)
     assert abs(W - o.flux[0]) <= 1e-5 * scale and abs(E - o.flux[1]) <= 1e-5 * scale
+    g.release()
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("grid", ["uniform", "stretched"])
+@pytest.mark.parametrize("stime", [0.0, 0.65])
+def test_time_step_both_criteria(grid, stime):
+    """grid_get_time_step (Sources/mod_Grid.F90:1026-1141) on the GPU with
+    criterion 1 (largest per-direction rate) and criterion 2 (sum of the
+    rates), for time-interpolated winds, against the FP64 oracle."""
+    print(f"case grid={grid} stime={stime}", flush=True)
+    p = _problem(44, grid, 23, 19, 11, 3)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    g.set_problem(p)
+    o = Oracle(p)
+    u = np.asfortranarray((1 - stime) * p.u1 + stime * p.u2)
+    v = np.asfortranarray((1 - stime) * p.v1 + stime * p.v2)
+    w = np.asfortranarray((1 - stime) * p.w1 + stime * p.w2)
+    for crit, cfl in ((1, 0.5), (2, 0.9)):
+        ref = o.time_step(u, v, w, p.vs, criterion=crit, cfl=cfl)
+        got = g.time_step(stime=stime, criterion=crit, cfl=cfl)
+        print(f"criterion {crit}: {got!r} vs {ref!r}", flush=True)
+        assert got == pytest.approx(ref, rel=1e-6)
+    # the two criteria differ on this problem (the test tells them apart)
+    assert o.time_step(u, v, w, p.vs, criterion=1, cfl=1.0) > 1.01 * o.time_step(u, v, w, p.vs, criterion=2, cfl=1.0)
     g.release()
