@@ -48,6 +48,7 @@ EXPORTS = (
     "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
+    "fall3d_ads_copy_probe",
     "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
@@ -100,6 +101,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_get_state": ([_vp, _vp, _vp], _int),
         "fall3d_ads_get_time_step": ([_vp, _dbl, _int, _dbl, _dp], _int),
         "fall3d_ads_step": ([_vp, _dbl, _dbl, _int], _int),
+        "fall3d_ads_copy_probe": ([_vp, _vp, ctypes.c_size_t, _vp], _int),
         "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
         "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
         "fall3d_ads_set_fluxes": ([_vp, _dp], _int),

@@ -632,3 +632,39 @@ hipError_t solve_1d(const Solve1DArgs &a, hipStream_t st) {
 }
 
 }  // namespace f3d
+
+// ---------------------------------------------------------------------------
+// Copy of known traffic for calibrating the HBM counters: every lane moves
+// 16 bytes per iteration with raw buffer loads and stores, the access width
+// of the sweeps' row fetches and stores. Chunks of at most 1 GiB per buffer
+// descriptor (32-bit byte offsets).
+// ---------------------------------------------------------------------------
+namespace f3d {
+
+__global__ void __launch_bounds__(256) k_copy_probe(const float *__restrict__ src, float *__restrict__ dst,
+                                                     uint32_t n4) {
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(src), (short)0, (int)(n4 * 16u), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (int)(n4 * 16u), 0x00020000);
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += gridDim.x * blockDim.x) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(q * 16u), 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(v, rd, (int)(q * 16u), 0, 0);
+  }
+}
+
+hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st) {
+  (void)hipGetLastError();
+  constexpr size_t kChunk = (size_t)1 << 28;  // floats (1 GiB)
+  for (size_t o = 0; o < n; o += kChunk) {
+    const size_t m = (n - o < kChunk) ? n - o : kChunk;
+    const uint32_t n4 = (uint32_t)(m / 4);
+    if (n4) hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, st, src + o, dst + o, n4);
+    for (size_t r = o + 4 * (size_t)n4; r < o + m; ++r) {  // 0-3 trailing floats
+      hipError_t e = hipMemcpyAsync(dst + r, src + r, sizeof(float), hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
+}
+
+}  // namespace f3d

@@ -113,7 +113,10 @@ Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
     t.cpt = C;
     t.S = n / C + 1;  // the last segment holds face n+1 (possibly no cell)
     t.wpl = (t.S + 63) / 64;
-    if (t.wpl > 4) t.S = 0;  // unsupported; caught by the caller
+    if (t.wpl > 4) {  // more than 256 lanes per line: unsupported, reported by the caller
+      t.S = 0;
+      return t;
+    }
     int lpb;  // lines per workgroup
     if (t.wpl <= 1) {
       t.wpl = 1;
@@ -257,8 +260,10 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
   make_sweep_args(c, dir, dt, stime, write_ghosts, v1, v2, kd, a);
 }
 
+// partials: per-workgroup partial sums of the launch (nullptr: the
+// context's own buffer of that sweep)
 int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *vel1, const float *vel2,
-              const float *kd, int write_ghosts, SweepOut *out) {
+              const float *kd, int write_ghosts, SweepOut *out, double *partials = nullptr) {
   {
     const int rc = ensure_dt(c, dt);
     if (rc) return rc;
@@ -266,6 +271,7 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   const Tiling &t = c->tl[dir];
   SweepArgs a;
   make_sweep_args(c, dir, dt, stime, write_ghosts, vel1, vel2, kd, a);
+  if (partials) a.partials = partials;
   dim3 grid = (dir == 0) ? dim3(t.gx, t.groups) : dim3(t.gx);
   dim3 block(t.block);
   {
@@ -278,10 +284,10 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   return FALL3D_ADS_OK;
 }
 
-void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks) {
+void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks, const double *partials = nullptr) {
   // partial slots: 0 ref beg, 1 ref end, 2 ledger beg, 3 ledger end
   const int dst[4] = {ACC_FLUX + 2 * dir, ACC_FLUX + 2 * dir + 1, ACC_LEDGER + 2 * dir, ACC_LEDGER + 2 * dir + 1};
-  j.part[j.count] = c->part[dir + 1];
+  j.part[j.count] = partials ? partials : c->part[dir + 1];
   j.nblocks[j.count] = nblocks;
   j.nslot[j.count] = 4;
   for (int s = 0; s < 4; ++s) j.dst[j.count][s] = dst[s];
@@ -470,6 +476,46 @@ int dropin_sweep(fall3d_ads_ctx *c, int dir, double *fbeg, double *fend, double 
   // the meteo buffers now hold per-call data
   c->meteo_ok = false;
   return FALL3D_ADS_OK;
+}
+
+// One F3D_time_step. part (nullptr: the context's buffers) gives the
+// per-workgroup partial-sum buffers of the source (0) and the x, y, z
+// sweeps (1..3).
+int step_impl(fall3d_ads_ctx *c, double dt, double stime, int iiter, double *const *part) {
+  int rc;
+  if ((rc = need_grid(c))) return rc;
+  if (!c->meteo_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_meteo has not been called");
+  if (c->learned && (rc = learned_pre_step(c, dt, stime))) return rc;
+  ReduceJob j;
+  memset(&j, 0, sizeof(j));
+  if (c->nsrc > 0) {
+    int nblk = 0;
+    double *sp = part ? part[0] : c->src_part;
+    {
+      ProfScope ps(c, KID_SOURCE);
+      hipError_t e = source_inject(c->chi, c->comp ? c->clo : nullptr, c->src_idx, c->src_rate, c->src_meas,
+                                   c->nsrc, dt, sp, &nblk, c->stream);
+      if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "source: %s", hipGetErrorString(e));
+    }
+    j.part[j.count] = sp;
+    j.nblocks[j.count] = nblk;
+    j.nslot[j.count] = 1;
+    j.dst[j.count][0] = ACC_INJECTED;
+    j.count++;
+  }
+  SweepOut so;
+  const int order[3] = {(iiter % 2 == 0) ? 0 : 1, (iiter % 2 == 0) ? 1 : 0, 2};
+  for (int q = 0; q < 3; ++q) {
+    const int d = order[q];
+    const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
+    const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
+    const float *kd = d == 0 ? c->k1 : (d == 1 ? c->k2 : c->k3);
+    double *pp = part ? part[d + 1] : nullptr;
+    if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so, pp))) return rc;
+    add_sweep_jobs(j, c, d, so.nblocks, pp);
+  }
+  if (c->learned && (rc = learned_post_step(c))) return rc;
+  return run_finalize(c, j);
 }
 
 }  // namespace host
@@ -896,38 +942,7 @@ int fall3d_ads_get_time_step(fall3d_ads_ctx *c, double stime, int criterion, dou
 
 int fall3d_ads_step(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
   if (!c) return FALL3D_ADS_EINVAL;
-  int rc;
-  if ((rc = need_grid(c))) return rc;
-  if (!c->meteo_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_meteo has not been called");
-  if (c->learned && (rc = learned_pre_step(c, dt, stime))) return rc;
-  ReduceJob j;
-  memset(&j, 0, sizeof(j));
-  if (c->nsrc > 0) {
-    int nblk = 0;
-    {
-      ProfScope ps(c, KID_SOURCE);
-      hipError_t e = source_inject(c->chi, c->comp ? c->clo : nullptr, c->src_idx, c->src_rate, c->src_meas,
-                                   c->nsrc, dt, c->src_part, &nblk, c->stream);
-      if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "source: %s", hipGetErrorString(e));
-    }
-    j.part[j.count] = c->src_part;
-    j.nblocks[j.count] = nblk;
-    j.nslot[j.count] = 1;
-    j.dst[j.count][0] = ACC_INJECTED;
-    j.count++;
-  }
-  SweepOut so;
-  const int order[3] = {(iiter % 2 == 0) ? 0 : 1, (iiter % 2 == 0) ? 1 : 0, 2};
-  for (int q = 0; q < 3; ++q) {
-    const int d = order[q];
-    const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
-    const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
-    const float *kd = d == 0 ? c->k1 : (d == 1 ? c->k2 : c->k3);
-    if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so))) return rc;
-    add_sweep_jobs(j, c, d, so.nblocks);
-  }
-  if (c->learned && (rc = learned_post_step(c))) return rc;
-  return run_finalize(c, j);
+  return step_impl(c, dt, stime, iiter, nullptr);
 }
 
 int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int nsteps) {
@@ -991,6 +1006,13 @@ int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int n
     --left;
   }
   return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_copy_probe(const void *src, void *dst, size_t n, void *stream) {
+  if ((!src || !dst) && n) return FALL3D_ADS_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (copy_probe((const float *)src, (float *)dst, n, st) != hipSuccess) return FALL3D_ADS_EHIP;
+  return hipStreamSynchronize(st) == hipSuccess ? FALL3D_ADS_OK : FALL3D_ADS_EHIP;
 }
 
 int fall3d_ads_set_graph(fall3d_ads_ctx *c, int on) {

@@ -17,13 +17,18 @@ from . import ads
 from .problem import config1_wind_problem
 
 
-def make_pair(nx, ny, nz, nb, seed, dx=5000.0, dz=500.0, nsources=1, puff=True, state="fp32", **kw):
-    """Fine and coarse contexts of one seeded problem (fine nx x ny, coarse half)."""
-    pf = config1_wind_problem(nx, ny, nz, nb, dx, dz, seed, nsources=nsources, puff=puff, **kw)
-    pc = config1_wind_problem(nx, ny, nz, nb, dx, dz, seed, coarsen=2, nsources=nsources, puff=puff, **kw)
-    fine = ads.ADS(pf.nx, pf.ny, pf.nz, pf.nb, "minmod", "rk", 8, state)
+def make_pair(nx, ny, nz, nb, seed, dx=5000.0, dz=500.0, nsources=1, puff=True, state="fp32", real_kind=8,
+              **kw):
+    """Fine and coarse contexts of one seeded problem (fine nx x ny, coarse
+    half). real_kind 4 builds the host arrays in float32 (large grids)."""
+    dtype = np.float32 if real_kind == 4 else np.float64
+    pf = config1_wind_problem(nx, ny, nz, nb, dx, dz, seed, nsources=nsources, puff=puff, dtype=dtype, **kw)
+    fine = ads.ADS(pf.nx, pf.ny, pf.nz, pf.nb, "minmod", "rk", real_kind, state)
     fine.set_problem(pf)
-    coarse = ads.ADS(pc.nx, pc.ny, pc.nz, pc.nb, "minmod", "rk", 8, "fp32")
+    del pf
+    pc = config1_wind_problem(nx, ny, nz, nb, dx, dz, seed, coarsen=2, nsources=nsources, puff=puff, dtype=dtype,
+                              **kw)
+    coarse = ads.ADS(pc.nx, pc.ny, pc.nz, pc.nb, "minmod", "rk", real_kind, "fp32")
     coarse.set_problem(pc)
     return fine, coarse
 
@@ -76,7 +81,7 @@ def rel_l1(a, ref):
 
 
 def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", gain=1.0, fine_fp32=False,
-             gains=None, variants=None, **kw):
+             gains=None, variants=None, real_kind=8, dtype=None, **kw):
     """Run fine (2 nsteps steps), coarse plain and coarse learned (nsteps steps
     each) from the same initial state; compare the coarse column loads and
     ground deposits with the fine ones averaged onto the coarse grid.
@@ -84,9 +89,11 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", g
     the same way (when the truth uses the compensated state). gains: several
     correction gains at once (keys "learned_<gain>"); variants: name ->
     keyword arguments of ADS.set_learned (smin, gain, flux_limit).
-    Returns a dict of relative L1 errors."""
+    real_kind 4 (dtype float32) keeps the host arrays of large grids in
+    float32. Returns a dict of relative L1 errors."""
     nx, ny, nz, nb = size
-    fine, coarse = make_pair(nx, ny, nz, nb, seed, state=truth_state, **kw)
+    dtype = dtype or (np.float32 if real_kind == 4 else np.float64)
+    fine, coarse = make_pair(nx, ny, nz, nb, seed, state=truth_state, real_kind=real_kind, **kw)
     dt = coarse_dt(fine, coarse, cfl)
     coarse.release()
     dz = kw.get("dz", 500.0)
@@ -97,7 +104,7 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", g
     dep_f = _coarse_mean(af, nx, ny).sum(axis=-1)
     out = {"dt": dt}
     if fine_fp32:
-        g = make_pair(nx, ny, nz, nb, seed, state="fp32", **kw)
+        g = make_pair(nx, ny, nz, nb, seed, state="fp32", real_kind=real_kind, **kw)
         g[1].release()
         g = g[0]
         g.run(0.5 * dt, 2 * nsteps, 0)
@@ -114,9 +121,9 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", g
     else:
         runs += [(f"learned_{g}", net, {"smin": smin, "gain": g}) for g in gains]
     for name, learned, opts in runs:
-        pc = config1_wind_problem(nx, ny, nz, nb, kw.get("dx", 5000.0), dz, seed, coarsen=2,
+        pc = config1_wind_problem(nx, ny, nz, nb, kw.get("dx", 5000.0), dz, seed, coarsen=2, dtype=dtype,
                                   **{k: v for k, v in kw.items() if k not in ("dx", "dz")})
-        g = ads.ADS(pc.nx, pc.ny, pc.nz, pc.nb, "minmod", "rk", 8, "fp32")
+        g = ads.ADS(pc.nx, pc.ny, pc.nz, pc.nb, "minmod", "rk", real_kind, "fp32")
         g.set_problem(pc)
         del pc
         g.set_learned(learned, **opts) if learned is not None else g.set_learned(None)

@@ -225,6 +225,13 @@ int fall3d_ads_get_kernel_time(fall3d_ads_ctx *ctx, int kernel, double *total_ms
  * arrays it must read once), for roofline accounting. */
 int fall3d_ads_sweep_bytes(fall3d_ads_ctx *ctx, int dir, double *bytes);
 
+/* Measurement aid, no counterpart in the reference: copies n floats from
+ * src to dst (device pointers) with 16-byte buffer loads and stores, the
+ * access width of the sweeps, on stream (NULL: the null stream), and
+ * synchronizes. A kernel of known traffic (4n bytes read, 4n written) for
+ * calibrating the HBM byte counters. */
+int fall3d_ads_copy_probe(const void *src, void *dst, size_t n, void *stream);
+
 /* ---- 3. Ensemble group --------------------------------------------------
  * FALL3D runs ensemble members (Sources/mod_Ensemble.F90, task_SetEns) as
  * independent task_Fall3d copies, one per MPI communicator

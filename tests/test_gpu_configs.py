@@ -137,7 +137,7 @@ def test_config5_fp32_against_compensated_and_tile_shapes():
     p = config1_wind_problem(*C5, seed=1234, dtype=np.float32, **C5_KW)
     nsteps = 8
     res = {}
-    for name, state, tiling in (("fp32", "fp32", None), ("fp32_tiles", "fp32", (4, 1, 1)),
+    for name, state, tiling in (("fp32", "fp32", None), ("fp32_tiles", "fp32", (8, 1, 1)),
                                 ("fp32x2", "fp32x2", None)):
         g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 4, state)
         g.set_problem(p)

@@ -411,3 +411,15 @@ def test_time_step_both_criteria(grid, stime):
     # the two criteria differ on this problem (the test tells them apart)
     assert o.time_step(u, v, w, p.vs, criterion=1, cfl=1.0) > 1.01 * o.time_step(u, v, w, p.vs, criterion=2, cfl=1.0)
     g.release()
+
+
+def test_x_tiling_past_256_lanes_is_refused():
+    """nx = 1024 at 3 or 4 cells per lane needs more than 256 lanes per x line:
+    set_tiling returns an error and keeps the previous tiling."""
+    g = ads.ADS(1024, 4, 4, 1)
+    before = g.tiling(0)
+    for cpt in (3, 4):
+        with pytest.raises(ads.Fall3dAdsError, match="tiling"):
+            g.set_tiling(0, cells_per_thread=cpt)
+        assert g.tiling(0) == before
+    g.release()
