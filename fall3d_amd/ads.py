@@ -45,7 +45,7 @@ EXPORTS = (
     "fall3d_ads_update_horizontal_vcomps", "fall3d_ads_solve_along_x", "fall3d_ads_solve_along_y",
     "fall3d_ads_solve_along_z", "fall3d_ads_solve_1d", "fall3d_ads_release", "fall3d_ads_last_error",
     "fall3d_ads_set_meteo", "fall3d_ads_set_source", "fall3d_ads_set_source_metric", "fall3d_ads_set_state", "fall3d_ads_get_state",
-    "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_run", "fall3d_ads_get_fluxes",
+    "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_step_noalloc", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
     "fall3d_ads_copy_probe",
@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_get_state": ([_vp, _vp, _vp], _int),
         "fall3d_ads_get_time_step": ([_vp, _dbl, _int, _dbl, _dp], _int),
         "fall3d_ads_step": ([_vp, _dbl, _dbl, _int], _int),
+        "fall3d_ads_step_noalloc": ([_vp, _dbl, _dbl, _int], _int),
         "fall3d_ads_copy_probe": ([_vp, _vp, ctypes.c_size_t, _vp], _int),
         "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
         "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
@@ -387,6 +388,12 @@ class ADS:
 
     def step(self, dt, iiter, stime=0.0):
         self._chk(load_library().fall3d_ads_step(self._h, float(dt), float(stime), int(iiter)), "step")
+
+    def step_noalloc(self, dt, iiter, stime=0.0):
+        """fall3d_ads_step_noalloc: the same step with its partial sums in the
+        library's static per-GPU array (no device allocation)."""
+        self._chk(load_library().fall3d_ads_step_noalloc(self._h, float(dt), float(stime), int(iiter)),
+                  "step_noalloc")
 
     def run(self, dt, nsteps, iiter0=0, stime=0.0):
         self._chk(load_library().fall3d_ads_run(self._h, float(dt), float(stime), int(iiter0), int(nsteps)), "run")

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -478,6 +479,35 @@ int dropin_sweep(fall3d_ads_ctx *c, int dir, double *fbeg, double *fend, double 
   return FALL3D_ADS_OK;
 }
 
+// ---- allocation-free step ---------------------------------------------------
+// Per-workgroup partial sums of fall3d_ads_step_noalloc live in this array
+// (one copy per GPU, held until the process ends). Host code never uses the
+// symbol itself: its device address is looked up once per device with
+// hipGetSymbolAddress and kept in noalloc_ptr.
+constexpr size_t kNoAllocScratchDoubles = (size_t)1 << 21;  // 16 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex noalloc_mu;
+static double *noalloc_ptr[kMaxDevices] = {};
+
+// device address of this device's scratch array (the current device)
+int noalloc_scratch(fall3d_ads_ctx *c, double **out) {
+  int dev = -1;
+  HIPCHK(c, hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(c, FALL3D_ADS_EINVAL, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    double *p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(c, FALL3D_ADS_EHIP, "scratch symbol address: %s", hipGetErrorString(e));
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return FALL3D_ADS_OK;
+}
+
 // One F3D_time_step. part (nullptr: the context's buffers) gives the
 // per-workgroup partial-sum buffers of the source (0) and the x, y, z
 // sweeps (1..3).
@@ -561,6 +591,12 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(c, FALL3D_ADS_EHIP, "no HIP device available");
+  {
+    // device address of the allocation-free step's scratch array on this device
+    double *scr = nullptr;
+    const int rc = noalloc_scratch(c, &scr);
+    if (rc) return rc;
+  }
   c->nx = nx;
   c->ny = ny;
   c->nz = nz;
@@ -1006,6 +1042,34 @@ int fall3d_ads_run(fall3d_ads_ctx *c, double dt, double stime, int iiter0, int n
     --left;
   }
   return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_step_noalloc(fall3d_ads_ctx *c, double dt, double stime, int iiter) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  if (c->learned)
+    return fail(c, FALL3D_ADS_EINVAL, "the allocation-free step runs plain KT only (learned correction is set)");
+  if (c->prof || c->prof_every)
+    return fail(c, FALL3D_ADS_EINVAL, "the allocation-free step does not record profiling events");
+  double *scr = nullptr;
+  int rc;
+  if ((rc = noalloc_scratch(c, &scr))) return rc;
+  // partial rows (4 doubles) of the source and of each sweep's workgroups
+  size_t rows[4];
+  rows[0] = (size_t)(c->nsrc + 255) / 256;
+  for (int d = 0; d < 3; ++d) rows[d + 1] = (d == 0) ? (size_t)c->tl[0].gx * c->tl[0].groups : (size_t)c->tl[d].gx;
+  const size_t need = 4 * (rows[0] + rows[1] + rows[2] + rows[3]);
+  if (need > kNoAllocScratchDoubles)
+    return fail(c, FALL3D_ADS_EINVAL,
+                "the step needs %zu doubles of partial sums; the allocation-free entry point holds %zu "
+                "(use fall3d_ads_step, or fewer workgroups: more bins per workgroup)",
+                need, kNoAllocScratchDoubles);
+  double *part[4];
+  size_t off = 0;
+  for (int q = 0; q < 4; ++q) {
+    part[q] = scr + off;
+    off += 4 * rows[q];
+  }
+  return step_impl(c, dt, stime, iiter, part);
 }
 
 int fall3d_ads_copy_probe(const void *src, void *dst, size_t n, void *stream) {

@@ -175,6 +175,21 @@ int fall3d_ads_get_time_step(fall3d_ads_ctx *ctx, double stime, int criterion, d
  * z) on the context stream. Asynchronous. */
 int fall3d_ads_step(fall3d_ads_ctx *ctx, double dt, double stime, int iiter);
 
+/* fall3d_ads_step without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-workgroup partial sums of the step's source and sweep kernels go
+ * to a static device array of 2^21 doubles (16 MiB) per GPU, held until the
+ * process ends, instead of the context's buffers; the kernels write every
+ * partial they later reduce, so nothing needs clearing. A step that needs
+ * more partial rows than that is rejected with EINVAL (the message gives
+ * both sizes); so is a context with a learned correction set or profiling
+ * on. Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (two contexts on different streams, or the same context from two
+ * threads) race and must be serialized by the caller. Results are bitwise
+ * those of fall3d_ads_step. */
+int fall3d_ads_step_noalloc(fall3d_ads_ctx *ctx, double dt, double stime, int iiter);
+
 /* Enqueue nsteps steps starting at iteration iiter0 with fixed dt/stime.
  * Step pairs are replayed from a captured hipGraph (re-captured when dt,
  * stime, the source or the tiling change) unless graphs are switched off
