@@ -268,7 +268,13 @@ def ensemble(args):
         del p
     setup_s = time.perf_counter() - t0
     dts = np.array(dts)
-    tiling, ms = ens.autotune(dts, reps=3)
+    if args.tiling:  # x cells per lane, y and z bins per workgroup (profiling runs)
+        tx, ty, tz = (int(v.split("/")[0]) for v in args.tiling.split(","))
+        ens.set_tiling(tx, ty, tz)
+        tiling = ens.tiling()
+        ms = [1.0, 1.0, 1.0]
+    else:
+        tiling, ms = ens.autotune(dts, reps=3)
     m0 = [g.budget()["mass_cells"] for g in ens.members]
     ens.run(dts, args.warmup, 0)
     ens.synchronize()

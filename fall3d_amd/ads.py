@@ -36,7 +36,7 @@ TIME_MARCHING = {"euler": EULER, "rk": RK, "runge-kutta": RK}
 STATES = {"fp32": STATE_FP32, "fp32x2": STATE_FP32X2}
 
 BUDGET_MASS_VOLUME, BUDGET_MASS_CELLS, BUDGET_GROUND, BUDGET_LATERAL, BUDGET_INJECTED, BUDGET_LEDGER = 0, 1, 2, 3, 4, 5
-BUDGET_LEN = 11
+BUDGET_LEN = 12
 KERNELS = {"source": 0, "sweep_x": 1, "sweep_y": 2, "sweep_z": 3, "reduce": 4}
 
 # Every entry point the header declares (checked by the CPU tests).
@@ -416,7 +416,7 @@ class ADS:
         out = np.array(b[:])
         return {
             "mass_volume": out[0], "mass_cells": out[1], "ground": out[2], "lateral": out[3], "injected": out[4],
-            "ledger": out[5:11].copy(),
+            "ledger": out[5:11].copy(), "residual": out[11],
         }
 
     def set_stream(self, stream_handle):

@@ -129,17 +129,19 @@ hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, h
 
 // dt/dxb(p) and 0.5 dt/dxb(p) in FP32 for the RK stage updates of one axis
 __global__ void k_dt_over_dxb(const double *__restrict__ dxb, int n3, double dt, double wfinal,
-                              float *__restrict__ tab) {
+                              float *__restrict__ tab, double *__restrict__ tab64) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n3) {
+    const double f = wfinal * dt / dxb[p];
     tab[p] = (float)(dt / dxb[p]);
     tab[n3 + p] = (float)(0.5 * dt / dxb[p]);
-    tab[2 * n3 + p] = (float)(wfinal * dt / dxb[p]);
+    tab[2 * n3 + p] = (float)f;
+    tab64[p] = f;
   }
 }
 
-__global__ void k_iter_table(const float *__restrict__ tab, const float *__restrict__ hdxp, int n, int nst,
-                             float *__restrict__ itab) {
+__global__ void k_iter_table(const float *__restrict__ tab, const float *__restrict__ hdxp,
+                             const double *__restrict__ dxb64, int n, int nst, float *__restrict__ itab) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= iter_rows(n)) return;
   const int t = row - kItabOffset;
@@ -154,21 +156,24 @@ __global__ void k_iter_table(const float *__restrict__ tab, const float *__restr
   r[1] = in(t - 3) ? hdtib[t - 3] : 0.0f;
   r[2] = in(t - 5) ? hdtib[t - 5] : 0.0f;
   r[3] = in(t - 7) ? dtib[t - 7] : 0.0f;
-  r[4] = ftib[cl(t - 2 * nst - 1)];
-  r[5] = r[6] = r[7] = 0.0f;
+  const int q = cl(t - 2 * nst - 1);
+  r[4] = ftib[q];
+  r[5] = (float)dxb64[q];  // weight of q's rounding residual
+  r[6] = r[7] = 0.0f;
 }
 
-hipError_t iter_table(const float *tab, const float *hdxp, int n, int nst, float *itab, hipStream_t st) {
+hipError_t iter_table(const float *tab, const float *hdxp, const double *dxb64, int n, int nst, float *itab,
+                      hipStream_t st) {
   (void)hipGetLastError();
   const int rows = iter_rows(n);
-  hipLaunchKernelGGL(k_iter_table, dim3((rows + 255) / 256), dim3(256), 0, st, tab, hdxp, n, nst, itab);
+  hipLaunchKernelGGL(k_iter_table, dim3((rows + 255) / 256), dim3(256), 0, st, tab, hdxp, dxb64, n, nst, itab);
   return hipGetLastError();
 }
 
-hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, hipStream_t st) {
+hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, double *tab64, hipStream_t st) {
   (void)hipGetLastError();
   const double wfinal = (tm == 1) ? 1.0 : 1.0 / 6.0;
-  hipLaunchKernelGGL(k_dt_over_dxb, dim3((n3 + 255) / 256), dim3(256), 0, st, dxb64, n3, dt, wfinal, tab);
+  hipLaunchKernelGGL(k_dt_over_dxb, dim3((n3 + 255) / 256), dim3(256), 0, st, dxb64, n3, dt, wfinal, tab, tab64);
   return hipGetLastError();
 }
 

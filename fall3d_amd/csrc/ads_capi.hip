@@ -170,7 +170,11 @@ int refresh_tiling(fall3d_ads_ctx *c) {
     const size_t need = (d == 0) ? (size_t)t.gx * t.groups : (size_t)t.gx;
     if (c->part_cap[d + 1] < need) {
       dfree(c->part[d + 1]);
+      dfree(c->rpart[d + 1]);
+      c->part[d + 1] = c->rpart[d + 1] = nullptr;
+      c->part_cap[d + 1] = 0;
       HIPCHK(c, dalloc(&c->part[d + 1], need * 4));
+      HIPCHK(c, dalloc(&c->rpart[d + 1], need * 4));
       c->part_cap[d + 1] = need;
     }
   }
@@ -187,8 +191,9 @@ struct SweepOut {
 int ensure_dt(fall3d_ads_ctx *c, double dt) {
   if (c->dtib_ok && c->dtib_dt == dt) return FALL3D_ADS_OK;
   for (int d = 0; d < 3; ++d) {
-    hipError_t e = dt_over_dxb(c->dxb64[d], axis_n(c, d) + 3, dt, c->tm, c->dttab[d], c->stream);
-    if (e == hipSuccess) e = iter_table(c->dttab[d], c->hdxp[d], axis_n(c, d), c->tm == 1 ? 1 : 4, c->itab[d], c->stream);
+    hipError_t e = dt_over_dxb(c->dxb64[d], axis_n(c, d) + 3, dt, c->tm, c->dttab[d], c->dttab64[d], c->stream);
+    if (e == hipSuccess)
+      e = iter_table(c->dttab[d], c->hdxp[d], c->dxb64[d], axis_n(c, d), c->tm == 1 ? 1 : 4, c->itab[d], c->stream);
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "dt/dxb: %s", hipGetErrorString(e));
   }
   c->dtib_dt = dt;
@@ -230,6 +235,7 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
     a.dtib = c->dttab[dir];
     a.hdtib = c->dttab[dir] + n3;
     a.ftib = c->dttab[dir] + 2 * n3;
+    a.ftib64 = c->dttab64[dir];
     a.itab = c->itab[dir];
   }
   a.dxb64 = c->dxb64[dir];
@@ -245,6 +251,7 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
   }
   a.acum = c->acum;
   a.partials = c->part[dir + 1];
+  a.rpartials = c->rpart[dir + 1];
   if (dir < 2 && (c->fout_on || (c->learned && c->learned_flux_limit > 0.0))) {
     // face fluxes: accumulated for training rows, overwritten for the
     // learned step's flux-relative clamp
@@ -261,10 +268,11 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
   make_sweep_args(c, dir, dt, stime, write_ghosts, v1, v2, kd, a);
 }
 
-// partials: per-workgroup partial sums of the launch (nullptr: the
-// context's own buffer of that sweep)
+// partials, rpartials: per-workgroup partial sums of the launch and of its
+// rounding residual (nullptr: the context's own buffers of that sweep)
 int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *vel1, const float *vel2,
-              const float *kd, int write_ghosts, SweepOut *out, double *partials = nullptr) {
+              const float *kd, int write_ghosts, SweepOut *out, double *partials = nullptr,
+              double *rpartials = nullptr) {
   {
     const int rc = ensure_dt(c, dt);
     if (rc) return rc;
@@ -273,6 +281,7 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   SweepArgs a;
   make_sweep_args(c, dir, dt, stime, write_ghosts, vel1, vel2, kd, a);
   if (partials) a.partials = partials;
+  if (rpartials) a.rpartials = rpartials;
   dim3 grid = (dir == 0) ? dim3(t.gx, t.groups) : dim3(t.gx);
   dim3 block(t.block);
   {
@@ -285,13 +294,20 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   return FALL3D_ADS_OK;
 }
 
-void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks, const double *partials = nullptr) {
+void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks, const double *partials = nullptr,
+                    const double *rpartials = nullptr) {
   // partial slots: 0 ref beg, 1 ref end, 2 ledger beg, 3 ledger end
   const int dst[4] = {ACC_FLUX + 2 * dir, ACC_FLUX + 2 * dir + 1, ACC_LEDGER + 2 * dir, ACC_LEDGER + 2 * dir + 1};
   j.part[j.count] = partials ? partials : c->part[dir + 1];
   j.nblocks[j.count] = nblocks;
   j.nslot[j.count] = 4;
   for (int s = 0; s < 4; ++s) j.dst[j.count][s] = dst[s];
+  j.count++;
+  // rounding residual of the FP32 state's final updates (zero for the compensated state)
+  j.part[j.count] = rpartials ? rpartials : c->rpart[dir + 1];
+  j.nblocks[j.count] = nblocks;
+  j.nslot[j.count] = 1;
+  j.dst[j.count][0] = ACC_RESIDUAL + dir;
   j.count++;
 }
 
@@ -431,7 +447,7 @@ int get_acc(fall3d_ads_ctx *c, int idx, double *vals, int n) {
   return FALL3D_ADS_OK;
 }
 
-// one drop-in sweep: flux pair in/out through acc[ACC_SCRATCH..]
+// one drop-in sweep: flux pair in/out through acc[ACC_FLUX + 2 dir ..]
 int dropin_sweep(fall3d_ads_ctx *c, int dir, double *fbeg, double *fend, double dt, void *cst, const void *kd,
                  const void *w1, const void *w2, double stime, const void *vsh, void *acum) {
   int rc;
@@ -509,8 +525,8 @@ int noalloc_scratch(fall3d_ads_ctx *c, double **out) {
 }
 
 // One F3D_time_step. part (nullptr: the context's buffers) gives the
-// per-workgroup partial-sum buffers of the source (0) and the x, y, z
-// sweeps (1..3).
+// per-workgroup partial-sum buffers of the source (0), the x, y, z sweeps
+// (1..3) and the sweeps' rounding residuals (4..6).
 int step_impl(fall3d_ads_ctx *c, double dt, double stime, int iiter, double *const *part) {
   int rc;
   if ((rc = need_grid(c))) return rc;
@@ -541,8 +557,9 @@ int step_impl(fall3d_ads_ctx *c, double dt, double stime, int iiter, double *con
     const float *v2 = d == 0 ? c->u2 : (d == 1 ? c->v2 : c->w2);
     const float *kd = d == 0 ? c->k1 : (d == 1 ? c->k2 : c->k3);
     double *pp = part ? part[d + 1] : nullptr;
-    if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so, pp))) return rc;
-    add_sweep_jobs(j, c, d, so.nblocks, pp);
+    double *rp = part ? part[d + 4] : nullptr;
+    if ((rc = run_sweep(c, d, dt, stime, v1, v2, kd, 1, &so, pp, rp))) return rc;
+    add_sweep_jobs(j, c, d, so.nblocks, pp, rp);
   }
   if (c->learned && (rc = learned_post_step(c))) return rc;
   return run_finalize(c, j);
@@ -640,6 +657,7 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     HIPCHK(c, dalloc(&c->hdxp[d], n + 3));
     HIPCHK(c, dalloc(&c->idxb[d], n + 3));
     HIPCHK(c, dalloc(&c->dttab[d], 3 * (n + 3)));
+    HIPCHK(c, dalloc(&c->dttab64[d], n + 3));
     HIPCHK(c, dalloc(&c->itab[d], (size_t)8 * iter_rows(n)));
   }
   const size_t ksz = (size_t)real_kind;
@@ -665,7 +683,8 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
   learned_release(c);
   void *ptrs[] = {c->chi_base, c->clo_base, c->acum, c->u1, c->u2, c->v1, c->v2, c->w1, c->w2, c->k1, c->k2, c->k3, c->vs,
                   c->src_idx, c->src_rate, c->src_meas, c->src_part, c->acc, c->part[1], c->part[2],
-                  c->part[3], c->red_part, c->dstage};  // part[0] aliases src_part
+                  c->part[3], c->rpart[1], c->rpart[2], c->rpart[3], c->red_part,
+                  c->dstage};  // part[0] aliases src_part
   for (void *p : ptrs) dfree(p);
   for (int d = 0; d < 3; ++d) {
     dfree(c->dxp64[d]);
@@ -673,6 +692,7 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
     dfree(c->hdxp[d]);
     dfree(c->idxb[d]);
     dfree(c->dttab[d]);
+    dfree(c->dttab64[d]);
     dfree(c->itab[d]);
   }
   free(c->hu);
@@ -1057,17 +1077,18 @@ int fall3d_ads_step_noalloc(fall3d_ads_ctx *c, double dt, double stime, int iite
   size_t rows[4];
   rows[0] = (size_t)(c->nsrc + 255) / 256;
   for (int d = 0; d < 3; ++d) rows[d + 1] = (d == 0) ? (size_t)c->tl[0].gx * c->tl[0].groups : (size_t)c->tl[d].gx;
-  const size_t need = 4 * (rows[0] + rows[1] + rows[2] + rows[3]);
+  // the sweeps' rows twice: their partial sums and their rounding residuals
+  const size_t need = 4 * (rows[0] + 2 * (rows[1] + rows[2] + rows[3]));
   if (need > kNoAllocScratchDoubles)
     return fail(c, FALL3D_ADS_EINVAL,
                 "the step needs %zu doubles of partial sums; the allocation-free entry point holds %zu "
                 "(use fall3d_ads_step, or fewer workgroups: more bins per workgroup)",
                 need, kNoAllocScratchDoubles);
-  double *part[4];
+  double *part[7];
   size_t off = 0;
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 7; ++q) {
     part[q] = scr + off;
-    off += 4 * rows[q];
+    off += 4 * rows[q < 4 ? q : q - 3];
   }
   return step_impl(c, dt, stime, iiter, part);
 }
@@ -1123,6 +1144,7 @@ int fall3d_ads_get_budget(fall3d_ads_ctx *c, double out[FALL3D_ADS_BUDGET_LEN]) 
       fabs(acc[ACC_FLUX + 5]);
   out[FALL3D_ADS_BUDGET_INJECTED] = acc[ACC_INJECTED];
   for (int s = 0; s < 6; ++s) out[FALL3D_ADS_BUDGET_LEDGER_W + s] = acc[ACC_LEDGER + s];
+  out[FALL3D_ADS_BUDGET_RESIDUAL] = acc[ACC_RESIDUAL + 0] + acc[ACC_RESIDUAL + 1] + acc[ACC_RESIDUAL + 2];
   return FALL3D_ADS_OK;
 }
 

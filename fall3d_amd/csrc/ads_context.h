@@ -18,7 +18,11 @@ namespace host {
 enum { KID_SOURCE = 0, KID_SWEEP_X = 1, KID_SWEEP_Y = 2, KID_SWEEP_Z = 3, KID_REDUCE = 4, KID_COUNT = 5 };
 
 // device accumulator layout
-enum { ACC_FLUX = 0, ACC_LEDGER = 6, ACC_INJECTED = 12, ACC_SCRATCH = 13, ACC_LEN = 16 };
+// ACC_RESIDUAL + d: FP32 state only, the mass the stored FP32 values lost
+// (or gained) to rounding in the final updates of sweep d, sum of
+// (exact - stored) dxb area. Every reduction job of a step owns its
+// destination slots: the jobs run concurrently in one launch.
+enum { ACC_FLUX = 0, ACC_LEDGER = 6, ACC_INJECTED = 12, ACC_RESIDUAL = 13, ACC_LEN = 16 };
 
 struct Tiling {
   int cpt = 0;      // cells per lane (x sweep) / bins per thread (y, z sweeps)
@@ -70,6 +74,7 @@ struct fall3d_ads_ctx {
   float *idxb[3] = {nullptr, nullptr, nullptr};
   float *dttab[3] = {nullptr, nullptr, nullptr};  // dt/dxb, 0.5 dt/dxb, final factor (for dtib_dt)
   float *itab[3] = {nullptr, nullptr, nullptr};   // per-iteration factors of the streaming sweeps
+  double *dttab64[3] = {nullptr, nullptr, nullptr};  // final-update factor w dt/dxb in FP64 (for dtib_dt)
   double dtib_dt = 0.0;
   bool dtib_ok = false;
   std::vector<double> hdxp64[3], hdxb64[3];
@@ -85,6 +90,7 @@ struct fall3d_ads_ctx {
   double *acc = nullptr;
   double *part[4] = {nullptr, nullptr, nullptr, nullptr};  // source, x, y, z
   size_t part_cap[4] = {0, 0, 0, 0};                        // in blocks
+  double *rpart[4] = {nullptr, nullptr, nullptr, nullptr};  // rounding-residual partials of x, y, z (1..3)
   double *red_part = nullptr;
   int red_cap = 0;
 

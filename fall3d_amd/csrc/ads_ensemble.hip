@@ -204,6 +204,13 @@ int build_tables(fall3d_ads_ensemble *e, const double *dt, const double *stime) 
         jb.nblocks = (d == 0) ? t.gx * t.groups : t.gx;
         jb.nslot = 4;
         jobs.push_back(jb);
+        BatchJob jr;
+        memset(&jr, 0, sizeof(jr));
+        jr.part = c->rpart[d + 1];
+        jr.dst[0] = c->acc + ACC_RESIDUAL + d;
+        jr.nblocks = jb.nblocks;
+        jr.nslot = 1;
+        jobs.push_back(jr);
       }
     }
     e->njobs = (int)jobs.size();

@@ -49,6 +49,8 @@ struct SweepArgs {
   const double *areaB;
   double *acum;              // ground accumulation (z sweep only), reference layout, FP64
   double *partials;          // [gridDim.x*gridDim.y][4]: ref flux beg/end, ledger beg/end
+  double *rpartials;         // [gridDim.x*gridDim.y][4]: slot 0 the FP32 rounding residual, slots 1-3 zero
+  const double *ftib64;      // final-update factor w dt/dxb(p) in FP64, p = 0..n+2 (FP32 state)
   float *fout;               // optional face mass flux output (x: u layout, y: v layout, per bin), or nullptr
   float fscale;              // fout += fscale * (G1 + 2 G2 + 2 G3 + G4) (RK4: dt/6; Euler: dt * G)
   int fout_add;              // 1: accumulate into fout; 0: overwrite (every interior face written)
@@ -63,19 +65,22 @@ struct SweepBatch {
 };
 
 // Table of one axis for time step dt (ads_aux.hip), 3 (n+3) floats:
-// [dt/dxb | 0.5 dt/dxb | w dt/dxb], w = 1/6 (RK4) or 1 (Euler, tm == 1).
-hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, hipStream_t st);
+// [dt/dxb | 0.5 dt/dxb | w dt/dxb], w = 1/6 (RK4) or 1 (Euler, tm == 1),
+// and the last one in FP64 (tab64, n+3 doubles).
+hipError_t dt_over_dxb(const double *dxb64, int n3, double dt, int tm, float *tab, double *tab64, hipStream_t st);
 // Per-iteration factors of the streaming sweep of an axis with n cells and
 // nst stages: row r (8 floats) belongs to iteration t = r - kItabOffset and
 // holds 0.5/dxp of face t-2, 0.5 dt/dxb of cells t-3 and t-5, dt/dxb of
-// cell t-7 (these three are 0 for cells outside 1..n) and the final factor
-// of cell t-2 nst-1 (indices clamped to 0..n+2). Iterations run from at
+// cell t-7 (these three are 0 for cells outside 1..n), the final factor
+// of cell q = t-2 nst-1 (indices clamped to 0..n+2) and dxb(q) in FP32
+// (slot 5, the weight of q's rounding residual). Iterations run from at
 // least 2 - kPF - 15 (> -kItabOffset) to n + 2 nst + 2 (the row fetched
 // one iteration ahead of the last).
 // tab: the dt_over_dxb table of the axis; hdxp: 0.5/dxp(f-1) at face f.
 constexpr int kItabOffset = 24;
 __host__ __device__ inline int iter_rows(int n) { return (n > 0 ? n : 0) + 2 * 4 + 3 + kItabOffset; }
-hipError_t iter_table(const float *tab, const float *hdxp, int n, int nst, float *itab, hipStream_t st);
+hipError_t iter_table(const float *tab, const float *hdxp, const double *dxb64, int n, int nst, float *itab,
+                      hipStream_t st);
 
 // Sweep launchers (ads_sweep_*.hip). cpt = cells per lane (x sweep) or
 // bins per thread (y and z sweeps, 1 or 2).

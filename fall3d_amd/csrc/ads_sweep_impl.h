@@ -106,6 +106,49 @@ __device__ __forceinline__ void block_partials(double v0, double v1, double v2, 
   }
 }
 
+// block_partials plus a fifth sum (the FP32 rounding residual) into its own
+// row layout: rpartials[block_id*4] = v4, slots 1-3 zero.
+__device__ __forceinline__ void block_partials5(double v0, double v1, double v2, double v3, double v4,
+                                                double *partials, double *rpartials, size_t block_id) {
+  __shared__ double red5[5][16];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const double v5[5] = {v0, v1, v2, v3, v4};
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const double r = wave_sum(v5[e]);
+    if (lane == 0) red5[e][wid] = r;
+  }
+  __syncthreads();
+  if (t < 8) {
+    double sum = 0.0;
+    if (t < 5) {
+      const int nw = (blockDim.x + 63) >> 6;
+      for (int w = 0; w < nw; ++w) sum += red5[t][w];
+    }
+    if (t < 4)
+      partials[block_id * 4 + t] = sum;
+    else
+      rpartials[block_id * 4 + (t - 4)] = sum;
+  }
+}
+
+// FP32-state final update with an exact FP64 residual: the cell's new value
+// c + F (gq - gq1) is formed in FP64 (F the FP64 final factor), stored
+// rounded to FP32, and what the rounding dropped, times the cell's weight w,
+// is added to res. Interior faces then cancel between their two cells to
+// FP64 rounding, whatever the spacing.
+__device__ __forceinline__ float final_update_fp64(float c, float gq, float gq1, double F, float w, bool count,
+                                                   double &res) {
+  const double c64 = fma((double)gq - (double)gq1, F, (double)c);
+  const float cn = (float)c64;
+  res = fma(c64 - (double)cn, count ? (double)w : 0.0, res);
+  // the accumulator is final here: without this the compiler defers the
+  // residual arithmetic of a whole unrolled block to its end and keeps every
+  // FP64 update alive until then (register spills)
+  __asm__ volatile("" : "+v"(res));
+  return cn;
+}
+
 typedef __attribute__((address_space(4))) const int cint;  // scalar-loaded tables
 
 // Member of an ensemble launch that owns workgroup bid: the last m with
@@ -307,6 +350,9 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
     }
   }
 
+  double res[BPT];  // FP32 state: rounding residual of this lane's final updates, per bin
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) res[j] = 0.0;
   // rings: cells (16) and low words (8, fetched kPF ahead of the final
   // update) per bin; raw face data (8) and face coefficients (8). The wind
   // lerp and K are shared by the bins of a thread; in the z sweep the
@@ -426,6 +472,11 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   const bool half = kHalf && (tstart + 8 <= 2 - kPF);
   const int tfirst = half ? tstart + 8 : tstart;
   rnext = itab[tfirst + kItabOffset];
+  // FP32 state: FP64 final factor of the cell each iteration updates
+  // (q = t - 2 NST - 1, clamped into the table), one iteration ahead
+  typedef __attribute__((address_space(4))) const double cdouble;
+  cdouble *fin64 = (cdouble *)a.ftib64;
+  double fnext = COMP ? 0.0 : fin64[clampi(tfirst - 2 * NST - 1, 0, n + 2)];
   // the first group's rows are all ghosts (the zeros above become them);
   // the rows of the next kGD-1 groups are fetched here
   unroll_range<1, kGD>([&](auto K) { fetch(tfirst + 4 * (decltype(K)::value - 1), K); });
@@ -434,8 +485,11 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   // compile-time constant: ring slots are static registers).
   auto step = [&](const int t, auto U) {
     constexpr int u = decltype(U)::value;
-    const f8 row = rnext;  // [0.5/dxp(t-2), 0.5dt/dxb(t-3), 0.5dt/dxb(t-5), dt/dxb(t-7), final(t-2NST-1)]
+    // [0.5/dxp(t-2), 0.5dt/dxb(t-3), 0.5dt/dxb(t-5), dt/dxb(t-7), final(q), dxb(q)], q = t-2NST-1
+    const f8 row = rnext;
     rnext = itab[t + 1 + kItabOffset];
+    const double fq = fnext;
+    if constexpr (!COMP) fnext = fin64[clampi(t - 2 * NST, 0, n + 2)];
     if constexpr ((u & 3) == 0) {
       // ---- group start: the rows fetched kGD groups ago (cells t-1..t+2,
       // faces and K cells t-2..t+1) into the rings, then the fetch for the
@@ -544,7 +598,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
           bstore(rl[j], vo, cpos(q), lv);
         } else {
           (void)vo;
-          out_f[(j * 4 + (u & 3)) * 64 + lane] = fmaf(dtq, gq - gq1, cn[j][cl]);
+          out_f[(j * 4 + (u & 3)) * 64 + lane] = final_update_fp64(cn[j][cl], gq, gq1, fq, row[5], q >= 1, res[j]);
         }
         if constexpr (FOUT) {  // face mass flux per unit area: faces q (1..n) and n+1
           const uint32_t fo = fpos(q), fo1 = fpos(q + 1);
@@ -588,11 +642,13 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       for (int j = 0; j < BPT; ++j) {
         const float rf = fmaf(2.0f, Plast[j], Glast[j]);
         const float gq = gh[j][(u - 2 * NST - 1) & 7], gq1 = gh[j][(u - 2 * NST) & 7];
+        // transfers through faces 1 and n+1: FP32 products for the compensated
+        // state, the raw face values for the FP32 state (scaled in FP64 below)
         if (t == 2 * NST + 1) cap_f[(0 * BPT + j) * 64 + lane] = rf;
-        if (t == 2 * NST + 2) cap_f[(1 * BPT + j) * 64 + lane] = dtq * gq;
+        if (t == 2 * NST + 2) cap_f[(1 * BPT + j) * 64 + lane] = COMP ? dtq * gq : gq;
         if (t == tend) {
           cap_f[(2 * BPT + j) * 64 + lane] = rf;
-          cap_f[(3 * BPT + j) * 64 + lane] = dtq * gq1;
+          cap_f[(3 * BPT + j) * 64 + lane] = COMP ? dtq * gq1 : gq1;
         }
       }
     }
@@ -611,9 +667,12 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   for (; t0 <= tend; t0 += 16)
     unroll_range<0, 16>([&](auto U) { step(t0 + decltype(U)::value, U); });
 
-  double rb = 0.0, re = 0.0, lb = 0.0, le = 0.0;
+  double rb = 0.0, re = 0.0, lb = 0.0, le = 0.0, rr = 0.0;
   if (lane_ok) {
     const double dt = (double)a.dt;
+    // FP32 state: the boundary transfers with the same FP64 factors as the
+    // final updates of cells 1 and n
+    const double f1 = COMP ? 1.0 : a.ftib64[1], fn = COMP ? 1.0 : a.ftib64[n];
 #pragma unroll
     for (int j = 0; j < BPT; ++j) {
       if (!bok[j]) continue;
@@ -621,15 +680,16 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       const float RFe = cap_f[(2 * BPT + j) * 64 + lane], trn = cap_f[(3 * BPT + j) * 64 + lane];
       rb += (double)RFb * area * dt;
       re += (double)RFe * area * dt;
-      lb -= (double)tl1 * a.dxb64[1] * area;
-      le += (double)trn * a.dxb64[n] * area;
+      lb -= (double)tl1 * f1 * a.dxb64[1] * area;
+      le += (double)trn * fn * a.dxb64[n] * area;
+      rr += res[j] * area;
       if constexpr (DIR == 2) {
         const size_t ai = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(t2 + 1) + (size_t)(ny + 4) * bins[j]);
         a.acum[ai] = a.acum[ai] + fabs((double)RFb) * dt;
       }
     }
   }
-  block_partials(rb, re, lb, le, a.partials, bid);
+  block_partials5(rb, re, lb, le, rr, a.partials, a.rpartials, bid);
 }
 
 // Workgroup order: consecutive workgroups are dispatched round-robin over
@@ -755,6 +815,8 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   // feed ghost positions, whose stage multipliers are zero, so their values
   // (the next line's, or the arrays' zeroed padding) do not matter.
   float up[C], um[C], kf[C], hb[C], db[C], fb[C];
+  double f64[C];  // FP32 state: FP64 final factors of the lane's cells
+  float wq[C];    // and their residual weights dxb
   float vbeg = 0.0f, vend = 0.0f;
   {
     float w1v[C], w2v[C], kk[C + 1], hx[C];
@@ -781,13 +843,24 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       hb[q] = in ? hb[q] : 0.0f;
       fb[q] = in ? fb[q] : 0.0f;
     }
+    if constexpr (!COMP) {
+      // FP64 final factors (0 beyond cell n and on idle lanes: those
+      // updates then keep the value exactly) and the cells' residual weights
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        const bool in = act && s0 + q <= n;
+        f64[q] = in ? a.ftib64[s0 + q] : 0.0;
+        wq[q] = (float)a.dxb64[s0 + q];
+      }
+    }
+
     if (act) {
       vbeg = s1 * a.vel1[vbase + 1] + s * a.vel2[vbase + 1];
       vend = s1 * a.vel1[vbase + (size_t)(n + 1)] + s * a.vel2[vbase + (size_t)(n + 1)];
     }
   }
 
-  double acc_rb = 0.0, acc_re = 0.0, acc_lb = 0.0, acc_le = 0.0;
+  double acc_rb = 0.0, acc_re = 0.0, acc_lb = 0.0, acc_le = 0.0, acc_res = 0.0;
   const int bfirst = a.bin0 + by * a.nbin;
   const int blast = min(bfirst + a.nbin, a.nbins);
 
@@ -952,21 +1025,33 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
     // ghost, so a lane whose cells end at n+2 or earlier stores them all
     // with wide stores (cells n+1, n+2 get the ghost value they hold); a
     // lane reaching further (the last segment) stores cells up to n only.
+    // transfers through faces 1 and n+1: FP32 products for the compensated
+    // state; for the FP32 state the raw face values, scaled in FP64 below
     float tl0 = 0.0f, trn = 0.0f;
+    double fl0 = 1.0, fln = 1.0;
+    double res = 0.0;
     float hv[C], lv[C];
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      const float tl = fb[q] * gh[q];
-      const float tr = fb[q] * gh[q + 1];
-      if (q == 0) tl0 = tl;
-      if (q == qn) trn = tr;
       if constexpr (COMP) {
+        const float tl = fb[q] * gh[q];
+        const float tr = fb[q] * gh[q + 1];
+        if (q == 0) tl0 = tl;
+        if (q == qn) trn = tr;
         hv[q] = cn[q];
         lv[q] = lo[q];
         comp_add(hv[q], lv[q], tl);
         comp_add(hv[q], lv[q], -tr);
       } else {
-        hv[q] = fmaf(fb[q], gh[q] - gh[q + 1], cn[q]);
+        if (q == 0) {
+          tl0 = gh[0];
+          fl0 = f64[0];
+        }
+        if (q == qn) {
+          trn = gh[q + 1];
+          fln = f64[q];
+        }
+        hv[q] = final_update_fp64(cn[q], gh[q], gh[q + 1], f64[q], wq[q], true, res);
       }
     }
     if (act) {
@@ -986,15 +1071,16 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
     if (act) {
       if (seg == 0) {
         acc_rb += (double)RFb * area * (double)dt;
-        acc_lb -= (double)tl0 * a.dxb64[1] * area;
+        acc_lb -= (double)tl0 * fl0 * a.dxb64[1] * area;
       }
       if (qn >= -1 && qn <= C - 2) acc_re += (double)RFe * area * (double)dt;  // holds face n+1
       if (qn >= 0 && qn <= C - 1) {  // holds cell n
-        acc_le += (double)trn * a.dxb64[n] * area;
+        acc_le += (double)trn * fln * a.dxb64[n] * area;
       }
+      acc_res += res * area;
     }
   }
-  block_partials(acc_rb, acc_re, acc_lb, acc_le, a.partials, (size_t)by * a.gx + bx);
+  block_partials5(acc_rb, acc_re, acc_lb, acc_le, acc_res, a.partials, a.rpartials, (size_t)by * a.gx + bx);
 }
 
 template <int C, int LIM, bool COMP, int NST, bool FOUT = false>

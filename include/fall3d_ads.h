@@ -74,7 +74,12 @@ typedef struct fall3d_ads_ctx fall3d_ads_ctx;
 #define FALL3D_ADS_BUDGET_LATERAL 3     /* |W|+|E|+|S|+|N|+|U| (Sources/mod_Grid.F90:1499-1500) */
 #define FALL3D_ADS_BUDGET_INJECTED 4    /* mass added by source injection */
 #define FALL3D_ADS_BUDGET_LEDGER_W 5    /* mass that left through W,E,S,N,D,U (RK-weighted, exact) */
-#define FALL3D_ADS_BUDGET_LEN 11
+/* FP32 state: the mass that rounding the sweeps' final updates to FP32 took
+ * from the stored field (sum of (FP64 update - stored FP32 value) dxb area);
+ * MASS_CELLS + ledger + RESIDUAL - INJECTED stays at its initial value to
+ * FP64 rounding. 0 with the compensated state. */
+#define FALL3D_ADS_BUDGET_RESIDUAL 11
+#define FALL3D_ADS_BUDGET_LEN 12
 
 /* ---------------------------------------------------------------------
  * Drop-in group (MODULE ADS)

@@ -423,3 +423,32 @@ def test_x_tiling_past_256_lanes_is_refused():
             g.set_tiling(0, cells_per_thread=cpt)
         assert g.tiling(0) == before
     g.release()
+
+
+@pytest.mark.parametrize("case", ["puff", "operational", "stretched"])
+def test_fp32_state_mass_closes_with_rounding_residual(case):
+    """FP32 state: every final update is formed in FP64 and stored rounded to
+    FP32; the mass that rounding moves is kept in the FP64 residual, so
+    mass + ledger + residual - injected stays at the initial mass to FP64
+    rounding (zero-flux puff, open operational domain with a source, and a
+    problem stretched on every axis)."""
+    if case == "puff":
+        p, nsteps = puff_problem(), 200
+    elif case == "operational":
+        p, nsteps = operational_problem(72, 48, 30, 4, seed=3), 60
+    else:
+        p, nsteps = _problem(12, "stretched", 40, 33, 18, 3, umax_scale=0.5), 40
+    print(f"case {case}", flush=True)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    g.set_problem(p)
+    dt = g.time_step(0.0, 2, 0.9 if case != "stretched" else 0.5)
+    m0 = g.budget()["mass_cells"]
+    g.run(dt, nsteps, 0)
+    b = g.budget()
+    g.release()
+    ref = m0 + b["injected"]
+    err = abs(b["mass_cells"] + b["ledger"].sum() + b["residual"] - ref) / ref
+    naive = abs(b["mass_cells"] + b["ledger"].sum() - ref) / ref
+    print(f"closure {err:.3e} (without the residual {naive:.3e}, residual {b['residual'] / ref:.3e})", flush=True)
+    assert err <= 1e-12
+    assert b["residual"] != 0.0
