@@ -48,7 +48,8 @@ EXPORTS = (
     "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_step_noalloc", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
-    "fall3d_ads_copy_probe",
+    "fall3d_ads_copy_probe", "fall3d_ads_check_state", "fall3d_ads_learned_fallbacks", "fall3d_ads_snapshot",
+    "fall3d_ads_restore", "fall3d_ens_statistics",
     "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
@@ -81,6 +82,14 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} not found: build it with `python -m fall3d_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 and
+    # the dynamic loader binds this library to whichever copy of that soname
+    # is loaded first. Loading torch first keeps torch's device use working
+    # in a process that also calls this library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     sig = {
         "fall3d_ads_initialize": ([ctypes.POINTER(_vp), _int, _int, _int, _int, _int, _int, _int, _int], _int),
@@ -103,6 +112,12 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_step": ([_vp, _dbl, _dbl, _int], _int),
         "fall3d_ads_step_noalloc": ([_vp, _dbl, _dbl, _int], _int),
         "fall3d_ads_copy_probe": ([_vp, _vp, ctypes.c_size_t, _vp], _int),
+        "fall3d_ads_check_state": ([_vp, _vp, _dp], _int),
+        "fall3d_ads_snapshot": ([_vp, _vp, ctypes.POINTER(ctypes.c_size_t), _dbl, ctypes.c_longlong], _int),
+        "fall3d_ads_restore": ([_vp, _vp, ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_longlong)], _int),
+        "fall3d_ens_statistics": ([_vp, _int, ctypes.c_long, _vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   _vp], _int),
+        "fall3d_ads_learned_fallbacks": ([_vp, _vp], _int),
         "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
         "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
         "fall3d_ads_set_fluxes": ([_vp, _dp], _int),
@@ -418,6 +433,46 @@ class ADS:
             "mass_volume": out[0], "mass_cells": out[1], "ground": out[2], "lateral": out[3], "injected": out[4],
             "ledger": out[5:11].copy(), "residual": out[11],
         }
+
+    def snapshot(self, time=0.0, iiter=0):
+        """Restart snapshot of the device state and accumulators (bytes)."""
+        lib = load_library()
+        n = ctypes.c_size_t(0)
+        self._chk(lib.fall3d_ads_snapshot(self._h, None, ctypes.byref(n), float(time), int(iiter)), "snapshot")
+        buf = np.empty(n.value, dtype=np.uint8)
+        self._chk(lib.fall3d_ads_snapshot(self._h, buf.ctypes.data, ctypes.byref(n), float(time), int(iiter)),
+                  "snapshot")
+        return buf.tobytes()
+
+    def restore(self, blob):
+        """Restore a snapshot; returns (time, iiter) stored in it."""
+        buf = np.frombuffer(blob, dtype=np.uint8)
+        t = ctypes.c_double(0.0)
+        it = ctypes.c_longlong(0)
+        self._chk(load_library().fall3d_ads_restore(self._h, buf.ctypes.data, buf.size, ctypes.byref(t),
+                                                    ctypes.byref(it)), "restore")
+        return t.value, int(it.value)
+
+    def save_restart(self, path, time=0.0, iiter=0):
+        with open(path, "wb") as f:
+            f.write(self.snapshot(time, iiter))
+
+    def load_restart(self, path):
+        with open(path, "rb") as f:
+            return self.restore(f.read())
+
+    def check_state(self):
+        """Non-finite and negative interior cells and the smallest finite value."""
+        counts = (ctypes.c_longlong * 2)()
+        mn = ctypes.c_double(0.0)
+        self._chk(load_library().fall3d_ads_check_state(self._h, counts, ctypes.byref(mn)), "check_state")
+        return {"nonfinite": int(counts[0]), "negative": int(counts[1]), "min": mn.value}
+
+    def learned_fallbacks(self):
+        """Learned steps that fell back to plain KT (non-finite network output)."""
+        n = ctypes.c_longlong(0)
+        self._chk(load_library().fall3d_ads_learned_fallbacks(self._h, ctypes.byref(n)), "learned_fallbacks")
+        return int(n.value)
 
     def set_stream(self, stream_handle):
         self._chk(load_library().fall3d_ads_set_stream(self._h, stream_handle), "set_stream")
@@ -752,3 +807,36 @@ def ADS_release():
     if _default is not None:
         _default.release()
         _default = None
+
+
+def ensemble_statistics(fields, percentiles=(), thresholds=(), logmean=True):
+    """Per-point statistics over ensemble members (fall3d_ens_statistics).
+
+    fields: torch tensor (nmem, npoints) of float64 on the GPU (member
+    fields on one grid). Returns a dict of GPU tensors: mean, logmean,
+    stddev, median, per (len(percentiles), npoints), prob (len(thresholds),
+    npoints)."""
+    import torch
+
+    if fields.dim() != 2 or fields.dtype != torch.float64 or not fields.is_cuda:
+        raise TypeError("fields: a 2-D float64 CUDA tensor (members, points) expected")
+    x = fields.contiguous()
+    nmem, npts = x.shape
+    dev = x.device
+    pv = torch.tensor(list(percentiles), dtype=torch.float64, device=dev)
+    tv = torch.tensor(list(thresholds), dtype=torch.float64, device=dev)
+    out = {k: torch.empty(npts, dtype=torch.float64, device=dev) for k in ("mean", "stddev", "median")}
+    out["logmean"] = torch.empty(npts, dtype=torch.float64, device=dev) if logmean else None
+    out["per"] = torch.empty((len(percentiles), npts), dtype=torch.float64, device=dev)
+    out["prob"] = torch.empty((len(thresholds), npts), dtype=torch.float64, device=dev)
+
+    def ptr(t):
+        return None if t is None or t.numel() == 0 else t.data_ptr()
+
+    rc = load_library().fall3d_ens_statistics(
+        x.data_ptr(), int(nmem), int(npts), ptr(pv), len(percentiles), ptr(tv), len(thresholds),
+        ptr(out["mean"]), ptr(out["logmean"]), ptr(out["stddev"]), ptr(out["median"]), ptr(out["per"]),
+        ptr(out["prob"]), torch.cuda.current_stream(dev).cuda_stream)
+    if rc != 0:
+        raise Fall3dAdsError(f"fall3d_ens_statistics failed ({rc})")
+    return out

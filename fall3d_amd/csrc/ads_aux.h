@@ -73,6 +73,9 @@ hipError_t mass(const float *chi, const float *clo, int nx, int ny, int nz, int 
                 const double *d2, const double *d3, int off, double *part, int part_cap, double *out,
                 hipStream_t st);
 hipError_t solve_1d(const Solve1DArgs &a, hipStream_t st);
+// non-finite cells, negative cells, smallest finite value of the interior -> out[0..2]
+hipError_t health(const float *chi, const float *clo, int nx, int ny, int nz, int nb, double *part, int part_cap,
+                  double *out, hipStream_t st);
 // n floats src -> dst with 16-byte buffer loads/stores (HBM counter calibration)
 hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st);
 

@@ -673,3 +673,84 @@ hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st) {
 }
 
 }  // namespace f3d
+
+// ---------------------------------------------------------------------------
+// Health of the state over the interior of every bin: non-finite cells,
+// negative cells and the smallest value (hi + lo for the compensated state)
+// ---------------------------------------------------------------------------
+namespace f3d {
+
+__global__ void __launch_bounds__(256) k_health(const float *__restrict__ chi, const float *__restrict__ clo, int nx,
+                                                int ny, int nz, int nb, double *__restrict__ part) {
+  const size_t ncell = (size_t)nx * ny * nz * nb;
+  double nonfin = 0.0, neg = 0.0, mn = INFINITY;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx) + 1;
+    size_t r = c / nx;
+    const int j = (int)(r % ny) + 1;
+    r /= ny;
+    const int k = (int)(r % nz) + 1;
+    const int b = (int)(r / nz);
+    const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
+    double v = (double)chi[ci];
+    if (clo) v += (double)clo[ci];
+    if (!isfinite(v)) {
+      nonfin += 1.0;
+    } else {
+      if (v < 0.0) neg += 1.0;
+      mn = fmin(mn, v);
+    }
+  }
+  __shared__ double red[3][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    nonfin += __shfl_down(nonfin, o, 64);
+    neg += __shfl_down(neg, o, 64);
+    mn = fmin(mn, __shfl_down(mn, o, 64));
+  }
+  if (lane == 0) {
+    red[0][wid] = nonfin;
+    red[1][wid] = neg;
+    red[2][wid] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, bneg = 0.0, m = INFINITY;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      a += red[0][w];
+      bneg += red[1][w];
+      m = fmin(m, red[2][w]);
+    }
+    part[blockIdx.x * 3 + 0] = a;
+    part[blockIdx.x * 3 + 1] = bneg;
+    part[blockIdx.x * 3 + 2] = m;
+  }
+}
+
+__global__ void k_health_final(const double *__restrict__ part, int nblk, double *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0, m = INFINITY;
+    for (int q = 0; q < nblk; ++q) {
+      a += part[q * 3 + 0];
+      b += part[q * 3 + 1];
+      m = fmin(m, part[q * 3 + 2]);
+    }
+    out[0] = a;
+    out[1] = b;
+    out[2] = m;
+  }
+}
+
+hipError_t health(const float *chi, const float *clo, int nx, int ny, int nz, int nb, double *part, int part_cap,
+                  double *out, hipStream_t st) {
+  (void)hipGetLastError();
+  const size_t n = (size_t)nx * ny * nz * nb;
+  int g = (int)((n + 255) / 256);
+  if (g > part_cap / 3) g = part_cap / 3;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_health, dim3(g), dim3(256), 0, st, chi, clo, nx, ny, nz, nb, part);
+  hipLaunchKernelGGL(k_health_final, dim3(1), dim3(64), 0, st, part, g, out);
+  return hipGetLastError();
+}
+
+}  // namespace f3d

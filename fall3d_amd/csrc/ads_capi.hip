@@ -1148,6 +1148,108 @@ int fall3d_ads_get_budget(fall3d_ads_ctx *c, double out[FALL3D_ADS_BUDGET_LEN]) 
   return FALL3D_ADS_OK;
 }
 
+// ---- restart snapshot ------------------------------------------------------
+// Layout (little-endian): 64-byte header, then the accumulators (ACC_LEN
+// doubles), acum (FP64), the FP32 state words and, for the compensated
+// state, its low words -- everything the device holds that a resumed run
+// reads, so that the resumed steps are bitwise those of an unbroken run.
+namespace {
+struct SnapHeader {
+  char magic[8];       // "F3DADSR1"
+  uint32_t version;    // 1
+  int32_t nx, ny, nz, nb, comp, limiter, tm;
+  double time;         // simulated time of the snapshot (caller's value)
+  int64_t iiter;       // next iteration (caller's value)
+  uint64_t acc_len;
+};
+static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
+constexpr char kSnapMagic[8] = {'F', '3', 'D', 'A', 'D', 'S', 'R', '1'};
+
+size_t snapshot_bytes(const fall3d_ads_ctx *c) {
+  return sizeof(SnapHeader) + sizeof(double) * ((size_t)ACC_LEN + c->nacum) +
+         sizeof(float) * c->ncell_c * (c->comp ? 2 : 1);
+}
+}  // namespace
+
+int fall3d_ads_snapshot(fall3d_ads_ctx *c, void *buf, size_t *bytes, double time, long long iiter) {
+  if (!c || !bytes) return FALL3D_ADS_EINVAL;
+  const size_t need = snapshot_bytes(c);
+  if (!buf) {
+    *bytes = need;
+    return FALL3D_ADS_OK;
+  }
+  if (*bytes < need) return fail(c, FALL3D_ADS_EINVAL, "snapshot buffer holds %zu bytes, %zu needed", *bytes, need);
+  SnapHeader h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, kSnapMagic, 8);
+  h.version = 1;
+  h.nx = c->nx;
+  h.ny = c->ny;
+  h.nz = c->nz;
+  h.nb = c->nb;
+  h.comp = c->comp;
+  h.limiter = c->limiter;
+  h.tm = c->tm;
+  h.time = time;
+  h.iiter = iiter;
+  h.acc_len = ACC_LEN;
+  char *p = (char *)buf;
+  memcpy(p, &h, sizeof(h));
+  p += sizeof(h);
+  HIPCHK(c, hipMemcpyAsync(p, c->acc, sizeof(double) * ACC_LEN, hipMemcpyDeviceToHost, c->stream));
+  p += sizeof(double) * ACC_LEN;
+  HIPCHK(c, hipMemcpyAsync(p, c->acum, sizeof(double) * c->nacum, hipMemcpyDeviceToHost, c->stream));
+  p += sizeof(double) * c->nacum;
+  HIPCHK(c, hipMemcpyAsync(p, c->chi, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
+  p += sizeof(float) * c->ncell_c;
+  if (c->comp) HIPCHK(c, hipMemcpyAsync(p, c->clo, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *bytes = need;
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_restore(fall3d_ads_ctx *c, const void *buf, size_t bytes, double *time, long long *iiter) {
+  if (!c || !buf) return FALL3D_ADS_EINVAL;
+  SnapHeader h;
+  if (bytes < sizeof(h)) return fail(c, FALL3D_ADS_EINVAL, "snapshot of %zu bytes has no header", bytes);
+  memcpy(&h, buf, sizeof(h));
+  if (memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != 1)
+    return fail(c, FALL3D_ADS_EINVAL, "not an ADS snapshot (magic or version)");
+  if (h.nx != c->nx || h.ny != c->ny || h.nz != c->nz || h.nb != c->nb)
+    return fail(c, FALL3D_ADS_EINVAL, "snapshot of a %d x %d x %d x %d grid, context is %d x %d x %d x %d", h.nx, h.ny,
+                h.nz, h.nb, c->nx, c->ny, c->nz, c->nb);
+  if (h.comp != c->comp) return fail(c, FALL3D_ADS_EINVAL, "snapshot state mode %d, context %d", h.comp, c->comp);
+  if (h.acc_len != (uint64_t)ACC_LEN) return fail(c, FALL3D_ADS_EINVAL, "snapshot accumulator layout differs");
+  const size_t need = snapshot_bytes(c);
+  if (bytes < need) return fail(c, FALL3D_ADS_EINVAL, "snapshot of %zu bytes, %zu expected", bytes, need);
+  const char *p = (const char *)buf + sizeof(h);
+  HIPCHK(c, hipMemcpyAsync(c->acc, p, sizeof(double) * ACC_LEN, hipMemcpyHostToDevice, c->stream));
+  p += sizeof(double) * ACC_LEN;
+  HIPCHK(c, hipMemcpyAsync(c->acum, p, sizeof(double) * c->nacum, hipMemcpyHostToDevice, c->stream));
+  p += sizeof(double) * c->nacum;
+  HIPCHK(c, hipMemcpyAsync(c->chi, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
+  p += sizeof(float) * c->ncell_c;
+  if (c->comp) HIPCHK(c, hipMemcpyAsync(c->clo, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (time) *time = h.time;
+  if (iiter) *iiter = h.iiter;
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_check_state(fall3d_ads_ctx *c, long long counts[2], double *min_value) {
+  if (!c || !counts) return FALL3D_ADS_EINVAL;
+  double *o = c->red_part + (c->red_cap - 3);
+  HIPCHK(c, health(c->chi, c->comp ? c->clo : nullptr, c->nx, c->ny, c->nz, c->nb, c->red_part, c->red_cap - 3, o,
+                   c->stream));
+  double h[3];
+  HIPCHK(c, hipMemcpyAsync(h, o, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  counts[0] = (long long)h[0];
+  counts[1] = (long long)h[1];
+  if (min_value) *min_value = h[2];
+  return FALL3D_ADS_OK;
+}
+
 int fall3d_ads_set_stream(fall3d_ads_ctx *c, void *stream) {
   if (!c) return FALL3D_ADS_EINVAL;
   HIPCHK(c, hipStreamSynchronize(c->stream));

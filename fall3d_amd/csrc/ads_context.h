@@ -128,6 +128,7 @@ struct fall3d_ads_ctx {
   unsigned long long *lbits = nullptr;  // per cell row, bit i-1 of word (i-1)/64: |c(i)| above the threshold
   int *lseg = nullptr;                   // 64-cell row segments next to an active tile (compacted list)
   unsigned *lcount = nullptr;            // counts of the three lists
+  unsigned *lbad = nullptr;              // [0] non-finite network output this step, [1] fallback steps
   double learned_smin = 0.0;
   double learned_gain = 1.0;
   double learned_flux_limit = 0.0;       // |dPhi| <= limit * |plain face flux| (0: off)

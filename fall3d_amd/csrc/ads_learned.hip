@@ -377,9 +377,12 @@ __global__ void __launch_bounds__(256) k_tile_flags(int axis, int nx, int ny, in
 }
 
 // inference: dPhi of every face of the active tiles
+// bad[0] is set when the network gives a non-finite output for an active
+// face (the clamps below would otherwise turn it into a finite value)
 __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedWeights *__restrict__ w,
                                                        float *__restrict__ dphi, const int *__restrict__ list,
-                                                       const unsigned *__restrict__ count) {
+                                                       const unsigned *__restrict__ count,
+                                                       unsigned *__restrict__ bad) {
   const int l = threadIdx.x & 63, col = l & 15;
   const long na = *count;
   const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -401,6 +404,7 @@ __global__ void __launch_bounds__(256) k_correct_faces(FaceArgs a, const PackedW
       TileAct ta;
       tile_forward(w, w1t, w2t, xb, ta);
       const float y = ta.y;
+      if (__ballot(act && !isfinite(y)) && l == 0) atomicOr(bad, 1u);
       if (act) {
         float F = S * geo.dxp * kOutScale * y;  // canonical direction (upwind -> downwind)
         // positivity: a face may move at most a quarter of its donor cell
@@ -511,12 +515,19 @@ __global__ void __launch_bounds__(256) k_touched_segments(const unsigned char *_
 // Each wave takes listed segments (uniform), one cell per lane; a cell none
 // of whose four faces lies in an active tile is left untouched: its
 // correction is zero. Every cell is updated by one thread, so the result
-// does not depend on the order of the list.
+// does not depend on the order of the list. When the network gave a
+// non-finite correction this step (bad[0]), nothing is applied: the state
+// stays the plain KT step, and bad[1] counts the fallback.
 __global__ void __launch_bounds__(256) k_apply_corrections(
     float *__restrict__ c, const float *__restrict__ dx, const float *__restrict__ dy, const float *__restrict__ px,
     const float *__restrict__ py, const unsigned char *__restrict__ flx, const unsigned char *__restrict__ fly,
     float lim, int nx, int ny, int nz, int nb, int nseg, const int *__restrict__ list,
-    const unsigned *__restrict__ count, const float *__restrict__ idxb_x, const float *__restrict__ idxb_y) {
+    const unsigned *__restrict__ count, const float *__restrict__ idxb_x, const float *__restrict__ idxb_y,
+    unsigned *__restrict__ bad) {
+  if (bad[0]) {  // uniform over the grid
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(bad + 1, 1u);
+    return;
+  }
   int ntx, nty;  // tiles per face row of each axis (multiples of 4)
   long rows_unused;
   face_space(0, nx, ny, nz, nb, ntx, rows_unused);
@@ -782,6 +793,7 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
   }
   // a fill kernel, not a memset: the step may be captured in a graph
   hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, c->stream, c->lcount, 3);
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, c->stream, c->lbad, 1);
   for (int axis = 0; axis < 2; ++axis) {
     FaceArgs a = face_args(c, axis, dt, stime, rel ? 0.0 : c->learned_smin);
     if (rel) a.smin_dev = c->lthr;
@@ -795,7 +807,7 @@ int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
                        dim3(256), 0, c->stream, axis, c->nx, c->ny,
                        c->nz, c->nb, c->lbits, c->lflag[axis], c->lact[axis], c->lcount + axis);
     hipLaunchKernelGGL(k_correct_faces, dim3(grid_for(nt)), dim3(256), 0, c->stream, a, c->learned->packed,
-                       c->dphi[axis], c->lact[axis], c->lcount + axis);
+                       c->dphi[axis], c->lact[axis], c->lcount + axis, c->lbad);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "face corrections: %s", hipGetErrorString(e));
   }
@@ -813,7 +825,7 @@ int learned_post_step(fall3d_ads_ctx *c) {
   hipLaunchKernelGGL(k_apply_corrections, dim3(blocks), dim3(256), 0, c->stream, c->chi, c->dphi[0], c->dphi[1],
                      lim > 0.0f ? c->fout[0] : c->dphi[0], lim > 0.0f ? c->fout[1] : c->dphi[1], c->lflag[0],
                      c->lflag[1], lim, c->nx, c->ny, c->nz, c->nb, nseg, c->lseg, c->lcount + 2, c->idxb[0],
-                     c->idxb[1]);
+                     c->idxb[1], c->lbad);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "apply corrections: %s", hipGetErrorString(e));
   return FALL3D_ADS_OK;
@@ -839,6 +851,8 @@ void learned_release(fall3d_ads_ctx *c) {
   c->lbits = nullptr;
   dfree(c->lcount);
   c->lcount = nullptr;
+  dfree(c->lbad);
+  c->lbad = nullptr;
   c->learned = nullptr;
 }
 
@@ -1068,6 +1082,8 @@ int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, doub
     if (ntiles(c, 0) > INT_MAX || ntiles(c, 1) > INT_MAX || (long)c->ny * c->nz * c->nb * ((c->nx + 63) / 64) > INT_MAX)
       return fail(c, FALL3D_ADS_EINVAL, "domain too large for the learned correction's tile list");
     if (!c->lcount) HIPCHK(c, dalloc(&c->lcount, 3));
+    if (!c->lbad) HIPCHK(c, dalloc(&c->lbad, 2));
+    HIPCHK(c, hipMemsetAsync(c->lbad, 0, 2 * sizeof(unsigned), c->stream));
     if (!c->lseg) HIPCHK(c, dalloc(&c->lseg, (size_t)c->ny * c->nz * c->nb * ((c->nx + 63) / 64)));
     if (!c->lbits) HIPCHK(c, dalloc(&c->lbits, (size_t)c->ny * c->nz * c->nb * ((c->nx + 63) / 64)));
     for (int a = 0; a < 2; ++a) {
@@ -1091,6 +1107,17 @@ int fall3d_ads_set_learned(fall3d_ads_ctx *c, fall3d_mlp *net, double smin, doub
   c->learned_flux_limit = net ? flux_limit : 0.0;
   c->version++;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FALL3D_ADS_OK;
+}
+
+int fall3d_ads_learned_fallbacks(fall3d_ads_ctx *c, long long *steps) {
+  if (!c || !steps) return FALL3D_ADS_EINVAL;
+  *steps = 0;
+  if (!c->lbad) return FALL3D_ADS_OK;
+  unsigned v[2] = {0u, 0u};
+  HIPCHK(c, hipMemcpyAsync(v, c->lbad, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *steps = (long long)v[1];
   return FALL3D_ADS_OK;
 }
 

@@ -211,6 +211,29 @@ int fall3d_ads_set_fluxes(fall3d_ads_ctx *ctx, const double flux[6]);
 /* FP64 budget, FALL3D_ADS_BUDGET_* indices (synchronizes). */
 int fall3d_ads_get_budget(fall3d_ads_ctx *ctx, double out[FALL3D_ADS_BUDGET_LEN]);
 
+/* Restart (the device-side counterpart of nc_IO_out_rst / nc_IO_read_rst,
+ * Sources/mod_nc_IO.F90:4524-5178, which store c with its halos, acum and
+ * the budget attributes). The snapshot is a byte blob: a 64-byte header
+ * (magic "F3DADSR1", version 1, int32 nx, ny, nz, nbins, state mode,
+ * limiter, time marching, double time, int64 iiter, uint64 number of
+ * accumulators), the FP64 accumulators (reference fluxes W,E,S,N,D,U,
+ * ledger, injected mass, rounding residuals), acum (FP64, its reference
+ * layout) and the device state words (FP32 c with ghosts; for the
+ * compensated state also its low words). Restoring it into a context of the
+ * same grid and state mode makes the following steps bitwise those of the
+ * uninterrupted run. buf NULL: *bytes receives the size. time and iiter are
+ * stored for the caller (restore returns them; either pointer may be NULL).
+ * No netCDF: the reference's restart files need netCDF-Fortran. */
+int fall3d_ads_snapshot(fall3d_ads_ctx *ctx, void *buf, size_t *bytes, double time, long long iiter);
+int fall3d_ads_restore(fall3d_ads_ctx *ctx, const void *buf, size_t bytes, double *time, long long *iiter);
+
+/* Health of the state, for checks after logged steps: counts[0] receives
+ * the number of non-finite interior cells (all bins), counts[1] the number
+ * of negative ones, min_value (may be NULL) the smallest finite value.
+ * No counterpart in the reference (its runs stop on I/O of NaN values
+ * only). Synchronizes. */
+int fall3d_ads_check_state(fall3d_ads_ctx *ctx, long long counts[2], double *min_value);
+
 /* Stream control: use an external hipStream_t (NULL = the library's own). */
 int fall3d_ads_set_stream(fall3d_ads_ctx *ctx, void *stream);
 int fall3d_ads_synchronize(fall3d_ads_ctx *ctx);
@@ -291,6 +314,26 @@ int fall3d_ads_ensemble_get_tiling(fall3d_ads_ensemble *ens, int out[3]);
 int fall3d_ads_ensemble_autotune(fall3d_ads_ensemble *ens, const double *dt, const double *stime, int reps,
                                  double *best_ms);
 
+/* ---- 3a. Ensemble statistics (Sources/task_PosEns.F90:319-441) ----------
+ * Statistics over the members of an ensemble at every grid point: x holds
+ * nmem member fields of npoints values, member-major (x[m npoints + p]),
+ * FP64, on the device (1 <= nmem <= 1024). Every output is a device array
+ * and may be NULL (not computed):
+ *   mean[p]                 sum_m x / nmem
+ *   logmean[p]              10^(sum_m log10 x / nmem)
+ *   stddev[p]               sqrt(sum_m (x - mean)^2 / (nmem - 1)) (the sum
+ *                           the reference's loop at :357-361 is written for;
+ *                           that loop keeps only the last member's term)
+ *   median[p]               r(int(nmem/2)), r the member values in ascending
+ *                           order (postp_rank_vector, Sources/mod_Postp.F90:185), 1-based
+ *   per[i npoints + p]      r(min(max(int(percentiles[i]/100 nmem), 1), nmem))
+ *   prob[i npoints + p]     100 #{m : x_m >= thresholds[i]} / nmem (percent)
+ * percentiles (nper values, in percent) and thresholds (nth) are device
+ * arrays. Runs on stream (NULL: the null stream) and synchronizes. */
+int fall3d_ens_statistics(const double *x, int nmem, long npoints, const double *percentiles, int nper,
+                          const double *thresholds, int nth, double *mean, double *logmean, double *stddev,
+                          double *median, double *per, double *prob, void *stream);
+
 /* ---- 4. Learned flux correction (coarse grid) ----------------------------
  * No counterpart in the reference. A per-face network (32 features -> 64 ->
  * 64 -> 1, GELU, BF16 weights and activations with FP32 accumulation on the
@@ -339,6 +382,11 @@ int fall3d_ads_get_face_flux(fall3d_ads_ctx *ctx, int axis, float *out);
  * applied correction (1: as predicted); flux_limit > 0 bounds each face's
  * correction by flux_limit times the plain step's flux through that face. */
 int fall3d_ads_set_learned(fall3d_ads_ctx *ctx, fall3d_mlp *net, double smin, double gain, double flux_limit);
+/* Plain KT is the fallback of the learned step: a step in which the network
+ * gives a non-finite output for any active face applies no correction at
+ * all (the state is then bitwise the plain KT step). steps receives the
+ * number of such steps since fall3d_ads_set_learned. */
+int fall3d_ads_learned_fallbacks(fall3d_ads_ctx *ctx, long long *steps);
 /* Interior of coarse := 2x2 horizontal means of fine (fine = 2x coarse). */
 int fall3d_ads_coarse_grain(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse);
 /* One training sample of the whole coarse grid: coarse := coarse-grained

@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "fall3d_ads.h")
 
 def _declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(fall3d_(?:ads|mlp)_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(fall3d_(?:ads|mlp|ens)_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_declares_the_binding_table():

@@ -179,3 +179,66 @@ def test_trained_correction_beats_plain_coarse_kt():
     assert res["learned"]["load_l1"] < 0.8 * res["plain"]["load_l1"]
     assert res["learned"]["deposit_l1"] < res["plain"]["deposit_l1"]
     net.release()
+
+
+@pytest.mark.parametrize("poison", ["nan_bias", "inf_weight", "none"])
+def test_learned_step_falls_back_to_plain_kt(poison):
+    """Plain KT is the learned step's fallback: a network whose output is not
+    finite on an active face (a NaN output bias, an infinite first-layer
+    weight) leaves the step bitwise equal to the plain KT step and counts one
+    fallback; a finite network applies its corrections and counts none."""
+    from fall3d_amd.problem import config1_wind_problem
+
+    print(f"case poison={poison}", flush=True)
+    p = config1_wind_problem(37, 29, 6, 2, seed=5, dx=5000.0, dz=1000.0, nsources=1, puff=True)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    g.set_problem(p)
+    dt = g.time_step(0.0, 2, 0.9)
+    g.run(dt, 4, 0)
+    c0, a0 = g.get_state()
+    g.set_learned(None)
+    g.step(dt, 4)
+    plain, _ = g.get_state()
+    params = ads.Network.initial_params(3)
+    if poison == "nan_bias":
+        params[-1] = np.nan
+    elif poison == "inf_weight":
+        params[5] = np.inf
+    net = ads.Network(params=params)
+    g.set_state(c0, a0)
+    g.set_learned(net, smin=1e-12, gain=1.0)
+    g.step(dt, 4)
+    got, _ = g.get_state()
+    fb = g.learned_fallbacks()
+    health = g.check_state()
+    g.release()
+    net.release()
+    print(f"fallbacks {fb}, health {health}", flush=True)
+    assert health["nonfinite"] == 0
+    if poison == "none":
+        assert fb == 0 and not np.array_equal(got, plain)
+    else:
+        assert fb == 1
+        assert np.array_equal(got, plain)
+
+
+def test_check_state_counts_bad_cells():
+    from fall3d_amd.problem import config1_wind_problem
+
+    p = config1_wind_problem(24, 20, 8, 3, seed=2, dx=5000.0, dz=1000.0, nsources=1, puff=True)
+    c = p.new_c()
+    c[5, 6, 4, 1] = np.nan
+    c[7, 3, 2, 0] = np.inf
+    c[3, 3, 3, 2] = -2.5e-3
+    c[10, 9, 5, 0] = -1e-9
+    c[0, 5, 5, 0] = np.nan  # a ghost cell: not part of the interior
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    g.set_problem(p)
+    g.set_state(c)
+    h = g.check_state()
+    g.release()
+    inner = c[2:-2, 2:-2, 2:-2, :].astype(np.float32).astype(np.float64)
+    fin = np.isfinite(inner)
+    assert h["nonfinite"] == int((~fin).sum()) == 2
+    assert h["negative"] == int((inner[fin] < 0).sum()) == 2
+    assert h["min"] == float(inner[fin].min())

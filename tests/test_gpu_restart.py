@@ -1,0 +1,68 @@
+"""Restart: a snapshot of the device state and accumulators, restored into
+a fresh context, resumes bitwise (the device-side counterpart of
+nc_IO_out_rst / nc_IO_read_rst, Sources/mod_nc_IO.F90:4524-5178).
+
+Run A: n1 steps, snapshot, n2 more steps. Run B: a new context with the same
+grid, meteo and source, the snapshot restored, n2 steps from the stored
+iteration. Concentration (ghosts included), deposit, reference fluxes and
+the whole FP64 budget must be equal bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fall3d_amd import ads
+from fall3d_amd.problem import operational_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("state", ["fp32", "fp32x2"])
+def test_resume_is_bitwise(state, tmp_path):
+    print(f"case: restart {state}", flush=True)
+    p = operational_problem(64, 48, 24, 3, seed=4)
+    a = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, state)
+    a.set_problem(p)
+    dt = a.time_step(0.0, 2, 0.9)
+    n1, n2 = 7, 6
+    a.run(dt, n1, 0)
+    path = os.path.join(tmp_path, "ads.rst")
+    a.save_restart(path, time=n1 * dt, iiter=n1)
+    a.run(dt, n2, n1)
+    ca, aa = a.get_state()
+    fa, ba = a.fluxes(), a.budget()
+    a.release()
+
+    b = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, state)
+    b.set_problem(p)  # grid, meteo and source; the state comes from the snapshot
+    t, it = b.load_restart(path)
+    assert t == n1 * dt and it == n1
+    b.run(dt, n2, it)
+    cb, ab = b.get_state()
+    fb, bb = b.fluxes(), b.budget()
+    b.release()
+    assert np.array_equal(ca, cb)
+    assert np.array_equal(aa, ab)
+    assert np.array_equal(fa, fb)
+    for k in ("mass_cells", "injected", "residual", "ground", "lateral"):
+        assert ba[k] == bb[k], k
+    assert np.array_equal(ba["ledger"], bb["ledger"])
+
+
+def test_restore_rejects_another_grid():
+    p = operational_problem(40, 32, 16, 2, seed=1)
+    a = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    a.set_problem(p)
+    blob = a.snapshot(0.0, 0)
+    a.release()
+    b = ads.ADS(p.nx + 1, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    with pytest.raises(ads.Fall3dAdsError, match="grid"):
+        b.restore(blob)
+    c = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32x2")
+    with pytest.raises(ads.Fall3dAdsError, match="state mode"):
+        c.restore(blob)
+    with pytest.raises(ads.Fall3dAdsError, match="magic"):
+        c.restore(b"\0" * 64 + blob[64:])
+    b.release()
+    c.release()
