@@ -30,10 +30,10 @@ LIB_PATH = os.environ.get("FALL3D_ADS_LIB") or os.path.join(_HERE, "lib", "libfa
 
 MINMOD, SUPERBEE, OSPRE = 1, 2, 3
 EULER, RK = 1, 2
-STATE_FP32, STATE_FP32X2 = 0, 1
+STATE_FP32, STATE_FP32X2, STATE_FP64 = 0, 1, 2
 LIMITERS = {"minmod": MINMOD, "superbee": SUPERBEE, "ospre": OSPRE}
 TIME_MARCHING = {"euler": EULER, "rk": RK, "runge-kutta": RK}
-STATES = {"fp32": STATE_FP32, "fp32x2": STATE_FP32X2}
+STATES = {"fp32": STATE_FP32, "fp32x2": STATE_FP32X2, "fp64": STATE_FP64}
 
 BUDGET_MASS_VOLUME, BUDGET_MASS_CELLS, BUDGET_GROUND, BUDGET_LATERAL, BUDGET_INJECTED, BUDGET_LEDGER = 0, 1, 2, 3, 4, 5
 BUDGET_LEN = 12

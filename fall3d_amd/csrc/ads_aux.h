@@ -62,6 +62,9 @@ hipError_t state_out(const float *hi, const float *lo, void *dst, int kind, size
 hipError_t to_f32(const void *src, int kind, float *dst, size_t n, hipStream_t st);
 hipError_t to_f64(const void *src, int kind, double *dst, size_t n, hipStream_t st);
 hipError_t transpose_to_f32(const void *src, int kind, float *dst, int A, int B, int K, hipStream_t st);
+hipError_t transpose_to_f64(const void *src, int kind, double *dst, int A, int B, int K, hipStream_t st);
+// FP64 device array -> n values of the host kind (4 or 8 bytes) in device memory
+hipError_t from_f64(const double *src, void *dst, int kind, size_t n, hipStream_t st);
 hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, hipStream_t st);
 hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double *rate_vol, const double *meas,
                          int nent, double dt, double *partials, int *nblocks, hipStream_t st);
@@ -69,13 +72,14 @@ hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st);
 hipError_t finalize_batch(const BatchJob *jobs, int njobs, hipStream_t st);
 hipError_t source_inject_batch(const SourceMember *tab, const int *bstart, int nmem, int nblocks, hipStream_t st);
 hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *out, hipStream_t st);
-hipError_t mass(const float *chi, const float *clo, int nx, int ny, int nz, int nb, const double *d1,
-                const double *d2, const double *d3, int off, double *part, int part_cap, double *out,
-                hipStream_t st);
+// state: FP64 c64 when given, else chi (+ clo)
+hipError_t mass(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,
+                const double *d1, const double *d2, const double *d3, int off, double *part, int part_cap,
+                double *out, hipStream_t st);
 hipError_t solve_1d(const Solve1DArgs &a, hipStream_t st);
 // non-finite cells, negative cells, smallest finite value of the interior -> out[0..2]
-hipError_t health(const float *chi, const float *clo, int nx, int ny, int nz, int nb, double *part, int part_cap,
-                  double *out, hipStream_t st);
+hipError_t health(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,
+                  double *part, int part_cap, double *out, hipStream_t st);
 // n floats src -> dst with 16-byte buffer loads/stores (HBM counter calibration)
 hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st);
 

@@ -45,15 +45,15 @@ __global__ void k_to_f64(const T *__restrict__ src, double *__restrict__ dst, si
 
 // dst[b + B*(a + A*k)] = src[a + A*(b + B*k)]  (swap the two fastest axes),
 // 32x32 tiles through LDS so both sides are coalesced.
-template <typename T>
-__global__ void k_transpose(const T *__restrict__ src, float *__restrict__ dst, int A, int B, int K) {
-  __shared__ float tile[32][33];
+template <typename T, typename O>
+__global__ void k_transpose(const T *__restrict__ src, O *__restrict__ dst, int A, int B, int K) {
+  __shared__ O tile[32][33];
   const int a0 = blockIdx.x * 32, b0 = blockIdx.y * 32;
   const int k = blockIdx.z;
   const size_t off = (size_t)A * B * k;
   for (int r = threadIdx.y; r < 32; r += blockDim.y) {
     const int a = a0 + threadIdx.x, b = b0 + r;
-    if (a < A && b < B) tile[r][threadIdx.x] = (float)src[off + a + (size_t)A * b];
+    if (a < A && b < B) tile[r][threadIdx.x] = (O)src[off + a + (size_t)A * b];
   }
   __syncthreads();
   for (int r = threadIdx.y; r < 32; r += blockDim.y) {
@@ -113,11 +113,38 @@ hipError_t transpose_to_f32(const void *src, int kind, float *dst, int A, int B,
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
   dim3 grid((A + 31) / 32, (B + 31) / 32, K), block(32, 8);
   if (kind == 8)
-    hipLaunchKernelGGL(k_transpose<double>, grid, block, 0, st, (const double *)src, dst, A, B, K);
+    hipLaunchKernelGGL((k_transpose<double, float>), grid, block, 0, st, (const double *)src, dst, A, B, K);
   else if (kind == 4)
-    hipLaunchKernelGGL(k_transpose<float>, grid, block, 0, st, (const float *)src, dst, A, B, K);
+    hipLaunchKernelGGL((k_transpose<float, float>), grid, block, 0, st, (const float *)src, dst, A, B, K);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t transpose_to_f64(const void *src, int kind, double *dst, int A, int B, int K, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  dim3 grid((A + 31) / 32, (B + 31) / 32, K), block(32, 8);
+  if (kind == 8)
+    hipLaunchKernelGGL((k_transpose<double, double>), grid, block, 0, st, (const double *)src, dst, A, B, K);
+  else if (kind == 4)
+    hipLaunchKernelGGL((k_transpose<float, double>), grid, block, 0, st, (const float *)src, dst, A, B, K);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_from_f64(const double *__restrict__ src, T *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (T)src[i];
+}
+
+hipError_t from_f64(const double *src, void *dst, int kind, size_t n, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  if (kind == 8)
+    hipLaunchKernelGGL(k_from_f64<double>, dim3(grid1d(n)), dim3(256), 0, st, src, (double *)dst, n);
+  else
+    hipLaunchKernelGGL(k_from_f64<float>, dim3(grid1d(n)), dim3(256), 0, st, src, (float *)dst, n);
   return hipGetLastError();
 }
 
@@ -491,9 +518,9 @@ hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *o
 // mass in the volume, Sources/mod_Grid.F90:1364-1375 (FP64)
 // ---------------------------------------------------------------------------
 
-__global__ void k_mass(const float *__restrict__ chi, const float *__restrict__ clo, int nx, int ny, int nz, int nb,
-                       const double *__restrict__ d1, const double *__restrict__ d2, const double *__restrict__ d3,
-                       int off, double *__restrict__ part) {
+__global__ void k_mass(const float *__restrict__ chi, const float *__restrict__ clo, const double *__restrict__ c64,
+                       int nx, int ny, int nz, int nb, const double *__restrict__ d1, const double *__restrict__ d2,
+                       const double *__restrict__ d3, int off, double *__restrict__ part) {
   const size_t ncell = (size_t)nx * ny * nz * nb;
   double m = 0.0;
   for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
@@ -504,8 +531,13 @@ __global__ void k_mass(const float *__restrict__ chi, const float *__restrict__ 
     const int k = (int)(r % nz) + 1;
     const int b = (int)(r / nz);
     const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
-    double cv = (double)chi[ci];
-    if (clo) cv += (double)clo[ci];
+    double cv;
+    if (c64) {
+      cv = c64[ci];
+    } else {
+      cv = (double)chi[ci];
+      if (clo) cv += (double)clo[ci];
+    }
     m += cv * d1[i + off] * d2[j + off] * d3[k + off];
   }
   block_sum_store(m, part);
@@ -526,15 +558,15 @@ __global__ void k_sum_final(const double *__restrict__ part, int nb, double *__r
   }
 }
 
-hipError_t mass(const float *chi, const float *clo, int nx, int ny, int nz, int nb, const double *d1,
-                const double *d2, const double *d3, int off, double *part, int part_cap, double *out,
-                hipStream_t st) {
+hipError_t mass(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,
+                const double *d1, const double *d2, const double *d3, int off, double *part, int part_cap,
+                double *out, hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
   const size_t n = (size_t)nx * ny * nz * nb;
   int g = (int)((n + 255) / 256);
   if (g > part_cap) g = part_cap;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL(k_mass, dim3(g), dim3(256), 0, st, chi, clo, nx, ny, nz, nb, d1, d2, d3, off, part);
+  hipLaunchKernelGGL(k_mass, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, d1, d2, d3, off, part);
   hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, part, g, out);
   return hipGetLastError();
 }
@@ -680,8 +712,9 @@ hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st) {
 // ---------------------------------------------------------------------------
 namespace f3d {
 
-__global__ void __launch_bounds__(256) k_health(const float *__restrict__ chi, const float *__restrict__ clo, int nx,
-                                                int ny, int nz, int nb, double *__restrict__ part) {
+__global__ void __launch_bounds__(256) k_health(const float *__restrict__ chi, const float *__restrict__ clo,
+                                                const double *__restrict__ c64, int nx, int ny, int nz, int nb,
+                                                double *__restrict__ part) {
   const size_t ncell = (size_t)nx * ny * nz * nb;
   double nonfin = 0.0, neg = 0.0, mn = INFINITY;
   for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
@@ -692,8 +725,13 @@ __global__ void __launch_bounds__(256) k_health(const float *__restrict__ chi, c
     const int k = (int)(r % nz) + 1;
     const int b = (int)(r / nz);
     const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
-    double v = (double)chi[ci];
-    if (clo) v += (double)clo[ci];
+    double v;
+    if (c64) {
+      v = c64[ci];
+    } else {
+      v = (double)chi[ci];
+      if (clo) v += (double)clo[ci];
+    }
     if (!isfinite(v)) {
       nonfin += 1.0;
     } else {
@@ -741,14 +779,14 @@ __global__ void k_health_final(const double *__restrict__ part, int nblk, double
   }
 }
 
-hipError_t health(const float *chi, const float *clo, int nx, int ny, int nz, int nb, double *part, int part_cap,
-                  double *out, hipStream_t st) {
+hipError_t health(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,
+                  double *part, int part_cap, double *out, hipStream_t st) {
   (void)hipGetLastError();
   const size_t n = (size_t)nx * ny * nz * nb;
   int g = (int)((n + 255) / 256);
   if (g > part_cap / 3) g = part_cap / 3;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL(k_health, dim3(g), dim3(256), 0, st, chi, clo, nx, ny, nz, nb, part);
+  hipLaunchKernelGGL(k_health, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, part);
   hipLaunchKernelGGL(k_health_final, dim3(1), dim3(64), 0, st, part, g, out);
   return hipGetLastError();
 }

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "ads_context.h"
+#include "ads_fp64.h"
 
 using namespace f3d;
 
@@ -161,6 +162,20 @@ Tiling choose_tiling(const fall3d_ads_ctx *c, int dir) {
 
 int refresh_tiling(fall3d_ads_ctx *c) {
   c->version++;
+  if (is_fp64(c)) {
+    // the FP64 sweeps: one thread per line, a capped grid of 256-thread workgroups
+    for (int d = 0; d < 3; ++d) {
+      const size_t need = (size_t)sweep64_blocks(axis_lines(c, d), c->nb);
+      if (c->part_cap[d + 1] < need) {
+        dfree(c->part[d + 1]);
+        c->part[d + 1] = nullptr;
+        c->part_cap[d + 1] = 0;
+        HIPCHK(c, dalloc(&c->part[d + 1], need * 4));
+        c->part_cap[d + 1] = need;
+      }
+    }
+    return FALL3D_ADS_OK;
+  }
   for (int d = 0; d < 3; ++d) {
     Tiling t = choose_tiling(c, d);
     if (d == 0 && (t.S <= 0 || t.cpt < 2 || !sweep_cpt_supported(t.cpt)))
@@ -294,6 +309,42 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   return FALL3D_ADS_OK;
 }
 
+// FP64 state: one sweep with the given FP64 winds (vel1, vel2: lerped by
+// stime), diffusion and, for z, settling velocity
+int run_sweep64(fall3d_ads_ctx *c, int dir, double dt, double stime, const double *vel1, const double *vel2,
+                const double *kd, SweepOut *out) {
+  Sweep64Args a;
+  memset(&a, 0, sizeof(a));
+  a.c = c->c64;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.nb = c->nb;
+  a.n = axis_n(c, dir);
+  a.tm = c->tm;
+  a.nlines = axis_lines(c, dir);
+  a.dt = dt;
+  a.stime = stime;
+  a.vel1 = vel1;
+  a.vel2 = vel2;
+  a.kd = kd;
+  a.vs = c->vsd;
+  a.dxp = c->dxp64[dir];
+  a.dxb = c->dxb64[dir];
+  a.areaA = c->dxb64[dir == 0 ? 1 : 0];
+  a.areaB = c->dxb64[dir == 2 ? 1 : 2];
+  a.acum = c->acum;
+  a.partials = c->part[dir + 1];
+  const int nblocks = sweep64_blocks(a.nlines, c->nb);
+  {
+    ProfScope ps(c, KID_SWEEP_X + dir);
+    hipError_t e = launch_sweep64(dir, c->limiter, a, nblocks, c->stream);
+    if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "FP64 sweep %d launch: %s", dir, hipGetErrorString(e));
+  }
+  out->nblocks = nblocks;
+  return FALL3D_ADS_OK;
+}
+
 void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks, const double *partials = nullptr,
                     const double *rpartials = nullptr) {
   // partial slots: 0 ref beg, 1 ref end, 2 ledger beg, 3 ledger end
@@ -303,6 +354,7 @@ void add_sweep_jobs(ReduceJob &j, const fall3d_ads_ctx *c, int dir, int nblocks,
   j.nslot[j.count] = 4;
   for (int s = 0; s < 4; ++s) j.dst[j.count][s] = dst[s];
   j.count++;
+  if (is_fp64(c)) return;  // no rounding residual to account for
   // rounding residual of the FP32 state's final updates (zero for the compensated state)
   j.part[j.count] = rpartials ? rpartials : c->rpart[dir + 1];
   j.nblocks[j.count] = nblocks;
@@ -342,61 +394,46 @@ size_t n_k3(const fall3d_ads_ctx *c) { return (size_t)(c->nz + 4) * c->nx * c->n
 size_t n_vs(const fall3d_ads_ctx *c) { return (size_t)c->nx * c->ny * (c->nz + 3) * c->nb; }
 
 // reference layout -> device layout for each coefficient field
-int up_u(fall3d_ads_ctx *c, const void *h, float *d) {
+// Field uploads: host kind -> FP32 device layout d and, for the FP64 state,
+// the same layout in FP64 (d64). Fields kept in the reference layout copy;
+// the others swap their two fastest axes (x first).
+int up_field(fall3d_ads_ctx *c, const void *h, size_t n, int A, int B, int K, float *d, double *d64) {
   void *s;
-  int rc = upload(c, h, n_u(c), &s);
+  int rc = upload(c, h, n, &s);
   if (rc) return rc;
-  HIPCHK(c, to_f32(s, c->kind, d, n_u(c), c->stream));
+  if (A == 0) {
+    HIPCHK(c, to_f32(s, c->kind, d, n, c->stream));
+    if (d64) HIPCHK(c, to_f64(s, c->kind, d64, n, c->stream));
+  } else {
+    HIPCHK(c, transpose_to_f32(s, c->kind, d, A, B, K, c->stream));
+    if (d64) HIPCHK(c, transpose_to_f64(s, c->kind, d64, A, B, K, c->stream));
+  }
   return FALL3D_ADS_OK;
 }
-int up_v(fall3d_ads_ctx *c, const void *h, float *d) {
-  void *s;
-  int rc = upload(c, h, n_v(c), &s);
-  if (rc) return rc;
-  HIPCHK(c, transpose_to_f32(s, c->kind, d, c->ny + 3, c->nx, c->nz, c->stream));
-  return FALL3D_ADS_OK;
+int up_u(fall3d_ads_ctx *c, const void *h, float *d, double *d64) { return up_field(c, h, n_u(c), 0, 0, 0, d, d64); }
+int up_v(fall3d_ads_ctx *c, const void *h, float *d, double *d64) {
+  return up_field(c, h, n_v(c), c->ny + 3, c->nx, c->nz, d, d64);
 }
-int up_w(fall3d_ads_ctx *c, const void *h, float *d) {
-  void *s;
-  int rc = upload(c, h, n_w(c), &s);
-  if (rc) return rc;
-  HIPCHK(c, transpose_to_f32(s, c->kind, d, c->nz + 3, c->nx * c->ny, 1, c->stream));
-  return FALL3D_ADS_OK;
+int up_w(fall3d_ads_ctx *c, const void *h, float *d, double *d64) {
+  return up_field(c, h, n_w(c), c->nz + 3, c->nx * c->ny, 1, d, d64);
 }
-int up_k1(fall3d_ads_ctx *c, const void *h) {
-  void *s;
-  int rc = upload(c, h, n_k1(c), &s);
-  if (rc) return rc;
-  HIPCHK(c, to_f32(s, c->kind, c->k1, n_k1(c), c->stream));
-  return FALL3D_ADS_OK;
-}
+int up_k1(fall3d_ads_ctx *c, const void *h) { return up_field(c, h, n_k1(c), 0, 0, 0, c->k1, c->k1d); }
 int up_k2(fall3d_ads_ctx *c, const void *h) {
-  void *s;
-  int rc = upload(c, h, n_k2(c), &s);
-  if (rc) return rc;
-  HIPCHK(c, transpose_to_f32(s, c->kind, c->k2, c->ny + 4, c->nx, c->nz, c->stream));
-  return FALL3D_ADS_OK;
+  return up_field(c, h, n_k2(c), c->ny + 4, c->nx, c->nz, c->k2, c->k2d);
 }
 int up_k3(fall3d_ads_ctx *c, const void *h) {
-  void *s;
-  int rc = upload(c, h, n_k3(c), &s);
-  if (rc) return rc;
-  HIPCHK(c, transpose_to_f32(s, c->kind, c->k3, c->nz + 4, c->nx * c->ny, 1, c->stream));
-  return FALL3D_ADS_OK;
+  return up_field(c, h, n_k3(c), c->nz + 4, c->nx * c->ny, 1, c->k3, c->k3d);
 }
-int up_vs(fall3d_ads_ctx *c, const void *h) {
-  void *s;
-  int rc = upload(c, h, n_vs(c), &s);
-  if (rc) return rc;
-  HIPCHK(c, to_f32(s, c->kind, c->vs, n_vs(c), c->stream));
-  return FALL3D_ADS_OK;
-}
+int up_vs(fall3d_ads_ctx *c, const void *h) { return up_field(c, h, n_vs(c), 0, 0, 0, c->vs, c->vsd); }
 
 int up_state(fall3d_ads_ctx *c, const void *h) {
   void *s;
   int rc = upload(c, h, c->ncell_c, &s);
   if (rc) return rc;
-  HIPCHK(c, state_in(s, c->kind, c->chi, c->comp ? c->clo : nullptr, c->ncell_c, c->stream));
+  if (is_fp64(c))
+    HIPCHK(c, to_f64(s, c->kind, c->c64, c->ncell_c, c->stream));
+  else
+    HIPCHK(c, state_in(s, c->kind, c->chi, c->comp ? c->clo : nullptr, c->ncell_c, c->stream));
   return FALL3D_ADS_OK;
 }
 
@@ -404,7 +441,10 @@ int down_state(fall3d_ads_ctx *c, void *h) {
   const size_t bytes = c->ncell_c * (size_t)c->kind;
   int rc = ensure_stage(c, bytes);
   if (rc) return rc;
-  HIPCHK(c, state_out(c->chi, c->comp ? c->clo : nullptr, c->dstage, c->kind, c->ncell_c, c->stream));
+  if (is_fp64(c))
+    HIPCHK(c, from_f64(c->c64, c->dstage, c->kind, c->ncell_c, c->stream));
+  else
+    HIPCHK(c, state_out(c->chi, c->comp ? c->clo : nullptr, c->dstage, c->kind, c->ncell_c, c->stream));
   HIPCHK(c, hipMemcpyAsync(h, c->dstage, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FALL3D_ADS_OK;
@@ -455,32 +495,45 @@ int dropin_sweep(fall3d_ads_ctx *c, int dir, double *fbeg, double *fend, double 
   if (!cst || !kd || !fbeg || !fend) return fail(c, FALL3D_ADS_EINVAL, "null argument");
   if ((rc = up_state(c, cst))) return rc;
   const float *vel1 = nullptr, *vel2 = nullptr, *kdd = nullptr;
+  const double *vd1 = nullptr, *vd2 = nullptr, *kd64 = nullptr;
   if (dir == 0) {
-    if ((rc = up_u(c, c->hu, c->u1))) return rc;
+    if ((rc = up_u(c, c->hu, c->u1, c->u1d))) return rc;
     if ((rc = up_k1(c, kd))) return rc;
     vel1 = vel2 = c->u1;
+    vd1 = vd2 = c->u1d;
     kdd = c->k1;
+    kd64 = c->k1d;
     stime = 0.0;
   } else if (dir == 1) {
-    if ((rc = up_v(c, c->hv, c->v1))) return rc;
+    if ((rc = up_v(c, c->hv, c->v1, c->v1d))) return rc;
     if ((rc = up_k2(c, kd))) return rc;
     vel1 = vel2 = c->v1;
+    vd1 = vd2 = c->v1d;
     kdd = c->k2;
+    kd64 = c->k2d;
     stime = 0.0;
   } else {
-    if ((rc = up_w(c, w1, c->w1))) return rc;
-    if ((rc = up_w(c, w2, c->w2))) return rc;
+    if ((rc = up_w(c, w1, c->w1, c->w1d))) return rc;
+    if ((rc = up_w(c, w2, c->w2, c->w2d))) return rc;
     if ((rc = up_k3(c, kd))) return rc;
     if ((rc = up_vs(c, vsh))) return rc;
     if ((rc = up_acum(c, acum))) return rc;
     vel1 = c->w1;
     vel2 = c->w2;
+    vd1 = c->w1d;
+    vd2 = c->w2d;
     kdd = c->k3;
+    kd64 = c->k3d;
   }
   double pair[2] = {*fbeg, *fend};
   if ((rc = set_acc(c, ACC_FLUX + 2 * dir, pair, 2))) return rc;
   SweepOut so;
-  if ((rc = run_sweep(c, dir, dt, stime, vel1, vel2, kdd, 1, &so))) return rc;
+  if (is_fp64(c)) {
+    if ((rc = ensure_dt(c, dt))) return rc;
+    if ((rc = run_sweep64(c, dir, dt, stime, vd1, vd2, kd64, &so))) return rc;
+  } else if ((rc = run_sweep(c, dir, dt, stime, vel1, vel2, kdd, 1, &so))) {
+    return rc;
+  }
   ReduceJob j;
   memset(&j, 0, sizeof(j));
   add_sweep_jobs(j, c, dir, so.nblocks);
@@ -539,8 +592,10 @@ int step_impl(fall3d_ads_ctx *c, double dt, double stime, int iiter, double *con
     double *sp = part ? part[0] : c->src_part;
     {
       ProfScope ps(c, KID_SOURCE);
-      hipError_t e = source_inject(c->chi, c->comp ? c->clo : nullptr, c->src_idx, c->src_rate, c->src_meas,
-                                   c->nsrc, dt, sp, &nblk, c->stream);
+      hipError_t e = is_fp64(c) ? source_inject64(c->c64, c->src_idx, c->src_s, c->src_vol, c->src_meas, c->nsrc,
+                                                  dt, sp, &nblk, c->stream)
+                                : source_inject(c->chi, c->comp ? c->clo : nullptr, c->src_idx, c->src_rate,
+                                                c->src_meas, c->nsrc, dt, sp, &nblk, c->stream);
       if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "source: %s", hipGetErrorString(e));
     }
     j.part[j.count] = sp;
@@ -551,6 +606,18 @@ int step_impl(fall3d_ads_ctx *c, double dt, double stime, int iiter, double *con
   }
   SweepOut so;
   const int order[3] = {(iiter % 2 == 0) ? 0 : 1, (iiter % 2 == 0) ? 1 : 0, 2};
+  if (is_fp64(c)) {
+    if ((rc = ensure_dt(c, dt))) return rc;
+    for (int q = 0; q < 3; ++q) {
+      const int d = order[q];
+      const double *v1 = d == 0 ? c->u1d : (d == 1 ? c->v1d : c->w1d);
+      const double *v2 = d == 0 ? c->u2d : (d == 1 ? c->v2d : c->w2d);
+      const double *kd = d == 0 ? c->k1d : (d == 1 ? c->k2d : c->k3d);
+      if ((rc = run_sweep64(c, d, dt, stime, v1, v2, kd, &so))) return rc;
+      add_sweep_jobs(j, c, d, so.nblocks);
+    }
+    return run_finalize(c, j);
+  }
   for (int q = 0; q < 3; ++q) {
     const int d = order[q];
     const float *v1 = d == 0 ? c->u1 : (d == 1 ? c->v1 : c->w1);
@@ -589,7 +656,7 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
   if (time_marching != 1 && time_marching != 2)
     return fail(c, FALL3D_ADS_EINVAL, "invalid time_marching %d", time_marching);
   if (real_kind != 4 && real_kind != 8) return fail(c, FALL3D_ADS_EINVAL, "invalid real kind %d", real_kind);
-  if (state_mode != 0 && state_mode != 1) return fail(c, FALL3D_ADS_EINVAL, "invalid state mode %d", state_mode);
+  if (state_mode < 0 || state_mode > 2) return fail(c, FALL3D_ADS_EINVAL, "invalid state mode %d", state_mode);
   {
     // The sweeps address each bin's concentration, and each wind, diffusion
     // and settling-velocity array, through a buffer descriptor whose record
@@ -626,10 +693,18 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
   c->own_stream = true;
   c->ncell_c = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * nbins;
   c->nacum = (size_t)(nx + 4) * (ny + 4) * nbins;
-  HIPCHK(c, dalloc(&c->chi_base, c->ncell_c + 2));
-  c->chi = c->chi_base + 2;
-  HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
-  if (c->comp) {
+  if (is_fp64(c)) {
+    HIPCHK(c, dalloc(&c->c64, c->ncell_c));
+    HIPCHK(c, hipMemsetAsync(c->c64, 0, c->ncell_c * sizeof(double), c->stream));
+    double **m64[10] = {&c->u1d, &c->u2d, &c->v1d, &c->v2d, &c->w1d, &c->w2d, &c->k1d, &c->k2d, &c->k3d, &c->vsd};
+    const size_t n64[10] = {n_u(c), n_u(c), n_v(c), n_v(c), n_w(c), n_w(c), n_k1(c), n_k2(c), n_k3(c), n_vs(c)};
+    for (int q = 0; q < 10; ++q) HIPCHK(c, dalloc(m64[q], n64[q]));
+  } else {
+    HIPCHK(c, dalloc(&c->chi_base, c->ncell_c + 2));
+    c->chi = c->chi_base + 2;
+    HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
+  }
+  if (c->comp == FALL3D_ADS_STATE_FP32X2) {
     HIPCHK(c, dalloc(&c->clo_base, c->ncell_c + 2));
     c->clo = c->clo_base + 2;
     HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_c * sizeof(float), c->stream));
@@ -684,7 +759,8 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
   void *ptrs[] = {c->chi_base, c->clo_base, c->acum, c->u1, c->u2, c->v1, c->v2, c->w1, c->w2, c->k1, c->k2, c->k3, c->vs,
                   c->src_idx, c->src_rate, c->src_meas, c->src_part, c->acc, c->part[1], c->part[2],
                   c->part[3], c->rpart[1], c->rpart[2], c->rpart[3], c->red_part,
-                  c->dstage};  // part[0] aliases src_part
+                  c->dstage, c->c64, c->u1d, c->u2d, c->v1d, c->v2d, c->w1d, c->w2d, c->k1d, c->k2d, c->k3d,
+                  c->vsd, c->src_s, c->src_vol};  // part[0] aliases src_part
   for (void *p : ptrs) dfree(p);
   for (int d = 0; d < 3; ++d) {
     dfree(c->dxp64[d]);
@@ -869,12 +945,12 @@ int fall3d_ads_set_meteo(fall3d_ads_ctx *c, const void *u1, const void *u2, cons
   if (!c) return FALL3D_ADS_EINVAL;
   if (!u1 || !v1 || !w1 || !k1 || !k2 || !k3 || !vs) return fail(c, FALL3D_ADS_EINVAL, "null meteo array");
   int rc;
-  if ((rc = up_u(c, u1, c->u1))) return rc;
-  if ((rc = up_u(c, u2 ? u2 : u1, c->u2))) return rc;
-  if ((rc = up_v(c, v1, c->v1))) return rc;
-  if ((rc = up_v(c, v2 ? v2 : v1, c->v2))) return rc;
-  if ((rc = up_w(c, w1, c->w1))) return rc;
-  if ((rc = up_w(c, w2 ? w2 : w1, c->w2))) return rc;
+  if ((rc = up_u(c, u1, c->u1, c->u1d))) return rc;
+  if ((rc = up_u(c, u2 ? u2 : u1, c->u2, c->u2d))) return rc;
+  if ((rc = up_v(c, v1, c->v1, c->v1d))) return rc;
+  if ((rc = up_v(c, v2 ? v2 : v1, c->v2, c->v2d))) return rc;
+  if ((rc = up_w(c, w1, c->w1, c->w1d))) return rc;
+  if ((rc = up_w(c, w2 ? w2 : w1, c->w2, c->w2d))) return rc;
   if ((rc = up_k1(c, k1))) return rc;
   if ((rc = up_k2(c, k2))) return rc;
   if ((rc = up_k3(c, k3))) return rc;
@@ -893,8 +969,10 @@ int fall3d_ads_set_source_metric(fall3d_ads_ctx *c, const void *s, const void *H
   dfree(c->src_rate);
   dfree(c->src_meas);
   dfree(c->src_part);
+  dfree(c->src_s);
+  dfree(c->src_vol);
   c->src_idx = nullptr;
-  c->src_rate = c->src_meas = c->src_part = nullptr;
+  c->src_rate = c->src_meas = c->src_part = c->src_s = c->src_vol = nullptr;
   c->part[0] = nullptr;
   c->nsrc = 0;
   c->version++;
@@ -911,7 +989,7 @@ int fall3d_ads_set_source_metric(fall3d_ads_ctx *c, const void *s, const void *H
       if (!(hm1(j) * hm2(j) * hm3(i, j) > 0.0))
         return fail(c, FALL3D_ADS_EINVAL, "non-positive map factor product at cell (%d,%d)", i, j);
   std::vector<size_t> idx;
-  std::vector<double> rate, meas;
+  std::vector<double> rate, meas, sval, volv;
   // Sources/mod_F3D.F90:1456-1473: c += dt*s/vol, vol = dX1_p dX2_p dX3_p Hm1 Hm2 Hm3
   for (int b = 0; b < c->nb; ++b)
     for (int k = 1; k <= nz; ++k)
@@ -924,6 +1002,8 @@ int fall3d_ads_set_source_metric(fall3d_ads_ctx *c, const void *s, const void *H
                              (hm1(j) * hm2(j) * hm3(i, j));
           idx.push_back((size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b)));
           rate.push_back(v / vol);
+          sval.push_back(v);
+          volv.push_back(vol);
           meas.push_back(c->hdxb64[0][i] * c->hdxb64[1][j] * c->hdxb64[2][k]);
         }
   c->nsrc = (int)idx.size();
@@ -937,6 +1017,13 @@ int fall3d_ads_set_source_metric(fall3d_ads_ctx *c, const void *s, const void *H
   HIPCHK(c, hipMemcpy(c->src_idx, idx.data(), sizeof(size_t) * idx.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->src_rate, rate.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->src_meas, meas.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
+  if (is_fp64(c)) {
+    // the FP64 state forms dt*s/vol as the reference does (not dt*(s/vol))
+    HIPCHK(c, dalloc(&c->src_s, idx.size()));
+    HIPCHK(c, dalloc(&c->src_vol, idx.size()));
+    HIPCHK(c, hipMemcpy(c->src_s, sval.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->src_vol, volv.data(), sizeof(double) * idx.size(), hipMemcpyHostToDevice));
+  }
   return FALL3D_ADS_OK;
 }
 
@@ -949,7 +1036,9 @@ int fall3d_ads_set_state(fall3d_ads_ctx *c, const void *cst, const void *acum) {
   int rc;
   if (cst) {
     if ((rc = up_state(c, cst))) return rc;
-  } else {  // zero field, no host transfer
+  } else if (is_fp64(c)) {  // zero field, no host transfer
+    HIPCHK(c, hipMemsetAsync(c->c64, 0, c->ncell_c * sizeof(double), c->stream));
+  } else {
     HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
     if (c->comp) HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_c * sizeof(float), c->stream));
   }
@@ -1068,6 +1157,7 @@ int fall3d_ads_step_noalloc(fall3d_ads_ctx *c, double dt, double stime, int iite
   if (!c) return FALL3D_ADS_EINVAL;
   if (c->learned)
     return fail(c, FALL3D_ADS_EINVAL, "the allocation-free step runs plain KT only (learned correction is set)");
+  if (is_fp64(c)) return fail(c, FALL3D_ADS_EINVAL, "the allocation-free step runs the FP32 states only");
   if (c->prof || c->prof_every)
     return fail(c, FALL3D_ADS_EINVAL, "the allocation-free step does not record profiling events");
   double *scr = nullptr;
@@ -1125,12 +1215,12 @@ int fall3d_ads_get_budget(fall3d_ads_ctx *c, double out[FALL3D_ADS_BUDGET_LEN]) 
   if ((rc = need_grid(c))) return rc;
   double *o = c->red_part + (c->red_cap - 2);
   // mass with mass-point spacings (dxp position p at index p+1) and with cell sizes
-  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->nx, c->ny, c->nz, c->nb, c->dxp64[0], c->dxp64[1],
+  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->c64, c->nx, c->ny, c->nz, c->nb, c->dxp64[0], c->dxp64[1],
                  c->dxp64[2], 1, c->red_part, c->red_cap - 2, o, c->stream));
   double mv = 0.0, mc = 0.0;
   HIPCHK(c, hipMemcpyAsync(&mv, o, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->nx, c->ny, c->nz, c->nb, c->dxb64[0], c->dxb64[1],
+  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->c64, c->nx, c->ny, c->nz, c->nb, c->dxb64[0], c->dxb64[1],
                  c->dxb64[2], 0, c->red_part, c->red_cap - 2, o, c->stream));
   HIPCHK(c, hipMemcpyAsync(&mc, o, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   double acc[ACC_LEN];
@@ -1165,9 +1255,13 @@ struct SnapHeader {
 static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
 constexpr char kSnapMagic[8] = {'F', '3', 'D', 'A', 'D', 'S', 'R', '1'};
 
+// bytes of the state words: FP32 hi (+ lo), or FP64
+size_t state_bytes(const fall3d_ads_ctx *c) {
+  return is_fp64(c) ? sizeof(double) * c->ncell_c : sizeof(float) * c->ncell_c * (c->comp ? 2 : 1);
+}
+
 size_t snapshot_bytes(const fall3d_ads_ctx *c) {
-  return sizeof(SnapHeader) + sizeof(double) * ((size_t)ACC_LEN + c->nacum) +
-         sizeof(float) * c->ncell_c * (c->comp ? 2 : 1);
+  return sizeof(SnapHeader) + sizeof(double) * ((size_t)ACC_LEN + c->nacum) + state_bytes(c);
 }
 }  // namespace
 
@@ -1200,9 +1294,13 @@ int fall3d_ads_snapshot(fall3d_ads_ctx *c, void *buf, size_t *bytes, double time
   p += sizeof(double) * ACC_LEN;
   HIPCHK(c, hipMemcpyAsync(p, c->acum, sizeof(double) * c->nacum, hipMemcpyDeviceToHost, c->stream));
   p += sizeof(double) * c->nacum;
-  HIPCHK(c, hipMemcpyAsync(p, c->chi, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
-  p += sizeof(float) * c->ncell_c;
-  if (c->comp) HIPCHK(c, hipMemcpyAsync(p, c->clo, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
+  if (is_fp64(c)) {
+    HIPCHK(c, hipMemcpyAsync(p, c->c64, state_bytes(c), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    HIPCHK(c, hipMemcpyAsync(p, c->chi, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
+    p += sizeof(float) * c->ncell_c;
+    if (c->comp) HIPCHK(c, hipMemcpyAsync(p, c->clo, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *bytes = need;
   return FALL3D_ADS_OK;
@@ -1227,9 +1325,13 @@ int fall3d_ads_restore(fall3d_ads_ctx *c, const void *buf, size_t bytes, double 
   p += sizeof(double) * ACC_LEN;
   HIPCHK(c, hipMemcpyAsync(c->acum, p, sizeof(double) * c->nacum, hipMemcpyHostToDevice, c->stream));
   p += sizeof(double) * c->nacum;
-  HIPCHK(c, hipMemcpyAsync(c->chi, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
-  p += sizeof(float) * c->ncell_c;
-  if (c->comp) HIPCHK(c, hipMemcpyAsync(c->clo, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
+  if (is_fp64(c)) {
+    HIPCHK(c, hipMemcpyAsync(c->c64, p, state_bytes(c), hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHK(c, hipMemcpyAsync(c->chi, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
+    p += sizeof(float) * c->ncell_c;
+    if (c->comp) HIPCHK(c, hipMemcpyAsync(c->clo, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (time) *time = h.time;
   if (iiter) *iiter = h.iiter;
@@ -1239,8 +1341,8 @@ int fall3d_ads_restore(fall3d_ads_ctx *c, const void *buf, size_t bytes, double 
 int fall3d_ads_check_state(fall3d_ads_ctx *c, long long counts[2], double *min_value) {
   if (!c || !counts) return FALL3D_ADS_EINVAL;
   double *o = c->red_part + (c->red_cap - 3);
-  HIPCHK(c, health(c->chi, c->comp ? c->clo : nullptr, c->nx, c->ny, c->nz, c->nb, c->red_part, c->red_cap - 3, o,
-                   c->stream));
+  HIPCHK(c, health(c->chi, c->comp ? c->clo : nullptr, c->c64, c->nx, c->ny, c->nz, c->nb, c->red_part,
+                   c->red_cap - 3, o, c->stream));
   double h[3];
   HIPCHK(c, hipMemcpyAsync(h, o, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1280,6 +1382,7 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
   int rc;
   if ((rc = need_grid(c))) return rc;
   if (!c->meteo_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_meteo has not been called");
+  if (is_fp64(c)) return fail(c, FALL3D_ADS_EINVAL, "the FP64 state has no tile shapes to tune");
   if (reps < 1) reps = 3;
   if ((rc = ensure_dt(c, dt))) return rc;
   // save
@@ -1397,6 +1500,7 @@ int fall3d_ads_get_tiling(fall3d_ads_ctx *c, int dir, int *cells_per_thread, int
 
 int fall3d_ads_set_tiling(fall3d_ads_ctx *c, int dir, int cells_per_thread, int columns, int bins_per_group) {
   if (!c || dir < 0 || dir > 2) return FALL3D_ADS_EINVAL;
+  if (is_fp64(c)) return fail(c, FALL3D_ADS_EINVAL, "the FP64 state has no tile shapes to set");
   if (dir == 0 && cells_per_thread && !sweep_cpt_supported(cells_per_thread))
     return fail(c, FALL3D_ADS_EINVAL, "unsupported cells per thread %d", cells_per_thread);
   // y/z: bins per thread, 1 or 2 (2 is not built for the compensated state)

@@ -57,6 +57,12 @@ struct fall3d_ads_ctx {
   // nx is a multiple of 4 (the sweeps read and write rows in 16-byte pieces)
   float *chi = nullptr, *clo = nullptr;
   float *chi_base = nullptr, *clo_base = nullptr;
+  // FP64 state (FALL3D_ADS_STATE_FP64): c64 holds the concentration in place
+  // of chi/clo, and the sweeps read FP64 copies of the fields below in the
+  // same device layouts (the FP32 copies remain for the CFL time step)
+  double *c64 = nullptr;
+  double *u1d = nullptr, *u2d = nullptr, *v1d = nullptr, *v2d = nullptr, *w1d = nullptr, *w2d = nullptr;
+  double *k1d = nullptr, *k2d = nullptr, *k3d = nullptr, *vsd = nullptr;
   double *acum = nullptr;
   size_t ncell_c = 0;  // (nx+4)(ny+4)(nz+4)nb
   size_t nacum = 0;    // (nx+4)(ny+4)nb
@@ -85,6 +91,7 @@ struct fall3d_ads_ctx {
   size_t *src_idx = nullptr;
   double *src_rate = nullptr, *src_meas = nullptr;
   double *src_part = nullptr;
+  double *src_s = nullptr, *src_vol = nullptr;  // FP64 state: rate and cell volume of each entry
 
   // accumulators and reduction scratch
   double *acc = nullptr;
@@ -164,6 +171,11 @@ inline void dfree(void *p) {
 }
 
 inline int axis_n(const fall3d_ads_ctx *c, int dir) { return dir == 0 ? c->nx : (dir == 1 ? c->ny : c->nz); }
+inline bool is_fp64(const fall3d_ads_ctx *c) { return c->comp == FALL3D_ADS_STATE_FP64; }
+// lines of a sweep along dir (per bin)
+inline long axis_lines(const fall3d_ads_ctx *c, int dir) {
+  return dir == 0 ? (long)c->ny * c->nz : (dir == 1 ? (long)c->nx * c->nz : (long)c->nx * c->ny);
+}
 
 // dt/dxb tables of the three axes for time step dt (stream-ordered)
 int ensure_dt(fall3d_ads_ctx *c, double dt);

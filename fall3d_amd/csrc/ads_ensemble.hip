@@ -337,6 +337,7 @@ int fall3d_ads_ensemble_create(fall3d_ads_ensemble **pens, int nmembers, const i
   if (!pens) return FALL3D_ADS_EINVAL;
   *pens = nullptr;
   if (nmembers < 1 || !dims) return FALL3D_ADS_EINVAL;
+  if (state_mode == FALL3D_ADS_STATE_FP64) return FALL3D_ADS_EINVAL;  // members run the FP32 states
   auto *e = new fall3d_ads_ensemble();
   e->nmem = nmembers;
   e->nb = nbins;

@@ -1127,6 +1127,7 @@ int fall3d_ads_learned_collect(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse, fal
   if (fine->nx != 2 * coarse->nx || fine->ny != 2 * coarse->ny || fine->nz != coarse->nz || fine->nb != coarse->nb)
     return nfail(n, FALL3D_ADS_EINVAL, "fine grid must be the 2x horizontal refinement of the coarse grid");
   if (n->cap == 0) return nfail(n, FALL3D_ADS_ESTATE, "network created without training capacity");
+  if (is_fp64(fine) || is_fp64(coarse)) return nfail(n, FALL3D_ADS_EINVAL, "training rows come from FP32 states");
   if (ntiles(coarse, 0) > INT_MAX || ntiles(coarse, 1) > INT_MAX)
     return nfail(n, FALL3D_ADS_EINVAL, "coarse domain too large for the face tiles");
   int rc;
@@ -1189,6 +1190,8 @@ int fall3d_ads_coarse_grain(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse) {
   if (!fine || !coarse) return FALL3D_ADS_EINVAL;
   if (fine->nx != 2 * coarse->nx || fine->ny != 2 * coarse->ny || fine->nz != coarse->nz || fine->nb != coarse->nb)
     return fail(coarse, FALL3D_ADS_EINVAL, "fine grid must be the 2x horizontal refinement of the coarse grid");
+  if (is_fp64(fine) || is_fp64(coarse))
+    return fail(coarse, FALL3D_ADS_EINVAL, "coarse graining runs on FP32 states (read an FP64 state with get_state)");
   HIPCHK(fine, hipStreamSynchronize(fine->stream));
   const size_t nc = (size_t)coarse->nx * coarse->ny * coarse->nz * coarse->nb;
   hipLaunchKernelGGL(k_coarse_grain, dim3((unsigned)std::min<size_t>((nc + 255) / 256, 65536)), dim3(256), 0,

@@ -57,9 +57,16 @@ typedef struct fall3d_ads_ctx fall3d_ads_ctx;
 #define FALL3D_ADS_RK 2
 /* device state of the concentration: plain FP32, or FP32 hi + FP32 lo
  * (compensated; telescoping flux updates are then exact to ~2^-48 and the
- * mass budget closes to FP64 rounding). */
+ * mass budget closes to FP64 rounding), or FP64 with FP64 fields: the
+ * reference's arithmetic statement by statement in double precision, for
+ * truth runs and real(8) drop-in use. The FP64 state supports the drop-in
+ * sweeps, set/get state, source, step/run (graphs included), time step,
+ * fluxes, budget, snapshot/restore and check_state; the tile controls, the
+ * allocation-free step, face-flux output and the learned correction are
+ * FP32-state features and return EINVAL for it. */
 #define FALL3D_ADS_STATE_FP32 0
 #define FALL3D_ADS_STATE_FP32X2 1
+#define FALL3D_ADS_STATE_FP64 2
 
 /* status codes */
 #define FALL3D_ADS_OK 0
