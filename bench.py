@@ -17,8 +17,11 @@ bounded sample, with sub-results under "sub":
                  central line source);
   ensemble    -- 64 members of uneven sizes (96-256 per horizontal axis, 40-60
                  levels, 6 bins) advanced in batched launches;
+  large_fp64  -- the large domain with the FP64 state (the reference's
+                 arithmetic in double precision; truth runs);
   learned     -- the learned coarse-grid correction on 512 x 384 x 80 x 20
-                 against the plain fine run, at equal simulated time.
+                 against the plain fine run, at equal simulated time; errors
+                 against the FP64 fine run.
 Each sub-result times --steps steps after --warmup warm-up steps.
 `--workload large|operational|ensemble|learned` runs one workload alone
 (the profiled commands use these).
@@ -47,7 +50,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--state", default="fp32", choices=["fp32", "fp32x2"])
+    ap.add_argument("--state", default="fp32", choices=["fp32", "fp32x2", "fp64"])
     ap.add_argument("--workload", default="all", choices=["all", "large", "operational", "ensemble", "learned"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle samples")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
@@ -152,7 +155,9 @@ def single_domain(args, which):
     g = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", kind, args.state)
     g.set_problem(p)
     dt = g.time_step(0.0, 2, 0.9)
-    if args.tiling:
+    if args.state == "fp64":
+        tiling = (0, 0, 0)  # one thread per line: no tile shapes
+    elif args.tiling:
         sx, sy, sz = args.tiling.split(",")
         tx = int(sx)
         ty, by = (int(v) for v in (sy + "/0").split("/")[:2])
@@ -208,7 +213,8 @@ def single_domain(args, which):
     del p
 
     return {
-        "metric": "Gcell*bin updates/s (ADS KT step, FP32 bulk, FP64 budgets)",
+        "metric": ("Gcell*bin updates/s (ADS KT step, FP64 state and fields)" if args.state == "fp64" else
+                   "Gcell*bin updates/s (ADS KT step, FP32 bulk, FP64 budgets)"),
         "value": round(value, 3),
         "unit": "Gcell*bin/s",
         "n_gpus": 1,
@@ -218,7 +224,7 @@ def single_domain(args, which):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.state == "fp32" else "f32x2",
+        "dtype": {"fp32": "f32", "fp32x2": "f32x2", "fp64": "f64"}[args.state],
         "data": "synthetic (seeded winds and line sources)",
         "config": {
             "workload": desc,
@@ -323,7 +329,7 @@ def learned_compare(args):
     coarse CFL step dt_c) with the plain fine run at its own CFL step dt_f:
     throughput at equal simulated time, and the column-load and deposit L1
     errors of plain and learned coarse runs against the fine run of the
-    compensated state after args.eval_steps coarse steps."""
+    FP64 state after args.eval_steps coarse steps."""
     import numpy as np
     import torch
 
@@ -379,7 +385,7 @@ def learned_compare(args):
     log(f"learned: fine {fine_ms:.3f} ms/step at dt {dt_f:.2f} s, learned coarse {coarse_ms:.3f} ms/step at "
         f"dt {dt_c:.2f} s -> {speedup:.2f}x at equal simulated time")
     del pf, pc
-    res = learned.evaluate(net, LARGE_SEED, LARGE, args.eval_steps, smin=1e-10, truth_state="fp32x2",
+    res = learned.evaluate(net, LARGE_SEED, LARGE, args.eval_steps, smin=1e-10, truth_state="fp64",
                            fine_fp32=True, gains=(1.0, 0.5), dtype=f32, real_kind=4, **kw)
     net.release()
     log(f"learned: evaluated {res}")
@@ -394,7 +400,7 @@ def learned_compare(args):
                    "training": {"grid": list(tr), "seeds": args.train_seeds, "rows": rows,
                                 "collect_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 2),
                                 "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-50:].mean())},
-                   "eval_coarse_steps": args.eval_steps, "errors_vs_fine_compensated": res,
+                   "eval_coarse_steps": args.eval_steps, "errors_vs_fine_fp64": res,
                    "autotuned_tiling": tiles},
         "roofline": None, "cpu_baseline": None,
     }
@@ -424,6 +430,11 @@ def main():
         log(f"operational: {sub['operational']['value']} Gcell*bin/s")
         sub["ensemble"] = ensemble(args)
         log(f"ensemble: {sub['ensemble']['value']} Gcell*bin/s")
+        # the FP64 state on the large domain (truth runs): a few steps, no CPU sample
+        a64 = argparse.Namespace(**dict(vars(args), state="fp64", no_cpu=True, tiling="",
+                                        steps=max(2, args.steps // 4), warmup=1))
+        sub["large_fp64"] = single_domain(a64, "large")
+        log(f"large fp64: {sub['large_fp64']['value']} Gcell*bin/s")
         sub["learned"] = learned_compare(args)
         out["sub"] = sub
     print(json.dumps(out))

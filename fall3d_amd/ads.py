@@ -49,7 +49,7 @@ EXPORTS = (
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
     "fall3d_ads_copy_probe", "fall3d_ads_check_state", "fall3d_ads_learned_fallbacks", "fall3d_ads_snapshot",
-    "fall3d_ads_restore", "fall3d_ens_statistics",
+    "fall3d_ads_restore", "fall3d_ens_statistics", "fall3d_ads_update_meteo", "fall3d_ads_get_meteo",
     "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
@@ -66,6 +66,33 @@ EXPORTS = (
 
 class Fall3dAdsError(RuntimeError):
     pass
+
+
+# fall3d_meteo_options / fall3d_meteo_grid / fall3d_meteo_corner of include/fall3d_ads.h
+class _MeteoOptions(ctypes.Structure):
+    _fields_ = [("map_h", ctypes.c_int), ("map_v", ctypes.c_int), ("modkh", ctypes.c_int), ("modkv", ctypes.c_int),
+                ("modv", ctypes.c_int), ("nbins", ctypes.c_int), ("kh0", ctypes.c_double), ("kv0", ctypes.c_double),
+                ("rams_cs", ctypes.c_double), ("dlon", ctypes.c_double), ("dlat", ctypes.c_double),
+                ("x3max", ctypes.c_double), ("s_time", ctypes.c_double), ("vset_fac", ctypes.c_double),
+                ("diam", ctypes.c_void_p), ("rho", ctypes.c_void_p), ("psi", ctypes.c_void_p),
+                ("cat", ctypes.c_void_p), ("aggregate", ctypes.c_void_p)]
+
+
+class _MeteoGrid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("Hm1_p", "Hm2_p", "Hm3_p", "dhdx_p", "dhdy_p", "z_c", "h_c", "X3_c",
+                                               "X3_p")]
+
+
+class _MeteoCorner(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p * 2) for n in ("uc", "vc", "wc", "tc", "tvc", "rhoc", "pblhc", "ustc", "monc")]
+
+
+METEO_MAP_H = {"cartesian": 0, "spherical": 1}
+METEO_MAP_V = {"cartesian": 0, "sigma": 1, "sigma_linear_decay": 2}
+METEO_KH = {"constant": 1, "cmaq": 2, "rams": 4}
+METEO_KV = {"constant": 1, "similarity": 3}
+METEO_VSET = {"arastopour": 1, "ganser": 2, "wilson": 3, "dellino": 4, "pfeiffer": 5, "dioguardi2017": 6,
+              "dioguardi2018": 7}
 
 
 _lib = None
@@ -118,6 +145,8 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ens_statistics": ([_vp, _int, ctypes.c_long, _vp, _int, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp,
                                    _vp], _int),
         "fall3d_ads_learned_fallbacks": ([_vp, _vp], _int),
+        "fall3d_ads_update_meteo": ([_vp, _vp, _vp, _vp], _int),
+        "fall3d_ads_get_meteo": ([_vp] * 11, _int),
         "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
         "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
         "fall3d_ads_set_fluxes": ([_vp, _dp], _int),
@@ -362,6 +391,66 @@ class ADS:
         self._chk(load_library().fall3d_ads_set_meteo(self._h, p(a["u1"]), p(a["u2"]), p(a["v1"]), p(a["v2"]),
                                                       p(a["w1"]), p(a["w2"]), p(a["k1"]), p(a["k2"]), p(a["k3"]),
                                                       p(a["vs"])), "set_meteo")
+
+    def update_meteo(self, corner, grid, opts):
+        """F3D_update_meteo_term on the device (fall3d_ads_update_meteo).
+
+        corner: dict of the corner fields, each a pair (first time, second
+        time): "uc", "vc", "wc", "tc", "rhoc" of shape (nx+1, ny+1, nz+1) and,
+        for similarity, "tvc" and the 2-D "pblhc", "ustc", "monc" (nx+1, ny+1).
+        grid: "Hm1_p" (ny), "Hm2_p" (ny+4), "Hm3_p" (nx, ny), and as the
+        options need them "dhdx_p", "dhdy_p", "z_c", "h_c", "X3_c", "X3_p".
+        opts: "map_h", "map_v", "modkh", "modkv", "modv" (codes or names of
+        METEO_*), "kh0", "kv0", "rams_cs", "dlon", "dlat", "x3max", "s_time",
+        "vset_fac" and the per-bin "diam", "rho", "psi", "cat", "aggregate"."""
+        keep = []
+
+        def arr(x, shape, name):
+            if x is None:
+                return None
+            a = self._arr(x, shape, name)
+            keep.append(a)
+            return a.ctypes.data
+
+        nx, ny, nz, nb = self.nx, self.ny, self.nz, self.nb
+        c3, c2 = (nx + 1, ny + 1, nz + 1), (nx + 1, ny + 1)
+        cm = _MeteoCorner()
+        for key, shp in (("uc", c3), ("vc", c3), ("wc", c3), ("tc", c3), ("tvc", c3), ("rhoc", c3),
+                         ("pblhc", c2), ("ustc", c2), ("monc", c2)):
+            pair = corner.get(key)
+            if pair is not None:
+                getattr(cm, key)[0] = arr(pair[0], shp, key)
+                getattr(cm, key)[1] = arr(pair[1], shp, key)
+        gm = _MeteoGrid()
+        for key, shp in (("Hm1_p", (ny,)), ("Hm2_p", (ny + 4,)), ("Hm3_p", (nx, ny)), ("dhdx_p", (nx, ny)),
+                         ("dhdy_p", (nx, ny)), ("z_c", c3), ("h_c", c2), ("X3_c", (nz + 1,)), ("X3_p", (nz,))):
+            setattr(gm, key, arr(grid.get(key), shp, key))
+        code = lambda tab, v: tab[v] if isinstance(v, str) else int(v)
+        bins = {k: np.ascontiguousarray(opts[k], dtype=np.float64) for k in ("diam", "rho", "psi")}
+        bins.update({k: np.ascontiguousarray(opts[k], dtype=np.int32) for k in ("cat", "aggregate")})
+        for k, v in bins.items():
+            if v.size != nb:
+                raise ValueError(f"{k}: {nb} bins expected")
+        om = _MeteoOptions(code(METEO_MAP_H, opts.get("map_h", 0)), code(METEO_MAP_V, opts.get("map_v", 0)),
+                           code(METEO_KH, opts.get("modkh", 1)), code(METEO_KV, opts.get("modkv", 1)),
+                           code(METEO_VSET, opts.get("modv", 2)), nb, float(opts.get("kh0", 0.0)),
+                           float(opts.get("kv0", 0.0)), float(opts.get("rams_cs", 0.2)), float(opts.get("dlon", 1.0)),
+                           float(opts.get("dlat", 1.0)), float(opts.get("x3max", 0.0)), float(opts.get("s_time", 0.5)),
+                           float(opts.get("vset_fac", 1.0)), bins["diam"].ctypes.data, bins["rho"].ctypes.data,
+                           bins["psi"].ctypes.data, bins["cat"].ctypes.data, bins["aggregate"].ctypes.data)
+        self._chk(load_library().fall3d_ads_update_meteo(self._h, ctypes.byref(cm), ctypes.byref(gm),
+                                                         ctypes.byref(om)), "update_meteo")
+        del keep
+
+    def get_meteo(self):
+        """The context's meteo in the layouts set_meteo takes (host kind)."""
+        nx, ny, nz, nb = self.nx, self.ny, self.nz, self.nb
+        shapes = {"u1": self.shape_u, "u2": self.shape_u, "v1": self.shape_v, "v2": self.shape_v,
+                  "w1": self.shape_w, "w2": self.shape_w, "k1": (nx + 4, ny, nz), "k2": (ny + 4, nx, nz),
+                  "k3": (nz + 4, nx, ny), "vs": self.shape_vs}
+        out = {k: np.zeros(v, dtype=self.dtype, order="F") for k, v in shapes.items()}
+        self._chk(load_library().fall3d_ads_get_meteo(self._h, *(out[k].ctypes.data for k in shapes)), "get_meteo")
+        return out
 
     def set_problem(self, prob):
         """Grid, meteo, source and initial state of a fall3d_amd.problem.Problem."""

@@ -1545,10 +1545,11 @@ int fall3d_ads_get_kernel_time(fall3d_ads_ctx *c, int kernel, double *total_ms, 
 int fall3d_ads_sweep_bytes(fall3d_ads_ctx *c, int dir, double *bytes) {
   if (!c || !bytes || dir < 0 || dir > 2) return FALL3D_ADS_EINVAL;
   const double cells = (double)c->nx * c->ny * c->nz;
-  const double st = c->comp ? 16.0 : 8.0;  // state read + write per cell and bin
+  const double st = c->comp ? 16.0 : 8.0;      // state read + write per cell and bin (FP32 pair or FP64)
+  const double fw = is_fp64(c) ? 8.0 : 4.0;     // bytes of a field value
   // coefficient fields read once per sweep: two wind snapshots + diffusion
-  double b = cells * (c->nb * st + 3.0 * 4.0);
-  if (dir == 2) b += cells * c->nb * 4.0;  // settling velocity per bin
+  double b = cells * (c->nb * st + 3.0 * fw);
+  if (dir == 2) b += cells * c->nb * fw;  // settling velocity per bin
   *bytes = b;
   return FALL3D_ADS_OK;
 }

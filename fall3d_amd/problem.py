@@ -418,3 +418,65 @@ def random_problem(nx, ny, nz, nb, seed=0, dx=1000.0, dy=1200.0, dz=150.0, wscal
                                    rng.uniform(0.0, 1e3, (nx, ny, nz, nb)), 0.0))
     return Problem(nx, ny, nz, nb, g, u, u.copy(order="F"), v, v.copy(order="F"), w, w2, k1, k2, k3, vs, c, s,
                    name=f"rand{nx}x{ny}x{nz}x{nb}s{seed}")
+
+
+def meteo_corner_problem(nx, ny, nz, nb, seed=0, map_v=0, map_h=0, modkh=2, modkv=3, modv=2, dx=5000.0, dy=5000.0,
+                         ztop=12000.0, s_time=0.5):
+    """Synthetic inputs of the meteo update (F3D_update_meteo_term): corner
+    fields at two meteo times (sheared winds with small-scale noise, a
+    stratified temperature and density, boundary-layer height, u* and a
+    Monin-Obukhov length of both signs, |L| < 10 included), terrain with
+    slopes for the sigma maps, map factors away from 1, and nb particle
+    bins (1 um - 2 mm, densities 800-2600 kg/m3, shape factors 0.5-1, one
+    aerosol bin, one aggregate bin). Returns (grid, corner, mgrid, opts):
+    grid the ADS spacing arrays, the rest the update's arguments."""
+    rng = np.random.default_rng(seed)
+    x3c = np.linspace(0.0, ztop, nz + 1)                       # sigma levels (corner heights above terrain)
+    x3p = 0.5 * (x3c[1:] + x3c[:-1])
+    X = (np.arange(nx + 1) * dx)[:, None]
+    Y = (np.arange(ny + 1) * dy)[None, :]
+    h_c = 400.0 + 300.0 * np.sin(X / (7.0 * dx)) * np.cos(Y / (5.0 * dy)) if map_v else np.zeros((nx + 1, ny + 1))
+    hm3_c = 1.0 - h_c / ztop
+    z_c = h_c[:, :, None] + x3c[None, None, :] * hm3_c[:, :, None]
+    corner = {k: [None, None] for k in ("uc", "vc", "wc", "tc", "tvc", "rhoc", "pblhc", "ustc", "monc")}
+    for t in (0, 1):
+        sh = 1.0 + 0.2 * t
+        zz = z_c / 1000.0
+        corner["uc"][t] = np.asfortranarray(sh * (5.0 + 1.5 * zz) + 0.002 * X[:, :, None] / dx + rng.normal(0, 0.5, z_c.shape))
+        corner["vc"][t] = np.asfortranarray(-2.0 + 0.8 * zz + 0.003 * Y[:, :, None] / dy + rng.normal(0, 0.5, z_c.shape))
+        corner["wc"][t] = np.asfortranarray(rng.normal(0.0, 0.05, z_c.shape))
+        tc = 288.0 + 2.0 * t - 6.5e-3 * z_c + rng.normal(0, 0.3, z_c.shape)
+        corner["tc"][t] = np.asfortranarray(tc)
+        corner["tvc"][t] = np.asfortranarray(tc * (1.0 + 0.61 * 0.005) + rng.normal(0, 0.05, z_c.shape))
+        corner["rhoc"][t] = np.asfortranarray(1.225 * np.exp(-z_c / 8500.0) * (1.0 + rng.normal(0, 0.01, z_c.shape)))
+        corner["pblhc"][t] = np.asfortranarray(rng.uniform(300.0, 1800.0, (nx + 1, ny + 1)))
+        corner["ustc"][t] = np.asfortranarray(rng.uniform(0.05, 0.6, (nx + 1, ny + 1)))
+        mon = rng.uniform(-400.0, 400.0, (nx + 1, ny + 1))
+        mon[::3, ::4] = rng.uniform(-12.0, 12.0, mon[::3, ::4].shape)  # |L| < 10 clamps
+        corner["monc"][t] = np.asfortranarray(mon)
+    hm3 = 1.0 - c2p_2d_np(h_c) / ztop
+    mgrid = {
+        "Hm1_p": rng.uniform(0.8, 1.0, ny) if map_h else np.ones(ny),
+        "Hm2_p": rng.uniform(0.95, 1.05, ny + 4) if map_h else np.ones(ny + 4),
+        "Hm3_p": np.asfortranarray(hm3) if map_v else np.ones((nx, ny), order="F"),
+        "dhdx_p": np.asfortranarray(rng.normal(0.0, 0.02, (nx, ny))),
+        "dhdy_p": np.asfortranarray(rng.normal(0.0, 0.02, (nx, ny))),
+        "z_c": np.asfortranarray(z_c), "h_c": np.asfortranarray(h_c), "X3_c": x3c, "X3_p": x3p,
+    }
+    g = uniform_grid(nx, ny, nz, dx, dy, ztop / nz)
+    diam = np.geomspace(1e-6, 2e-3, nb)
+    cat = np.ones(nb, dtype=np.int32)
+    agg = np.zeros(nb, dtype=np.int32)
+    if nb > 2:
+        cat[0] = 2          # an aerosol bin
+        agg[nb // 2] = 1    # an aggregate bin
+    opts = {"map_h": map_h, "map_v": map_v, "modkh": modkh, "modkv": modkv, "modv": modv,
+            "kh0": 250.0, "kv0": 15.0, "rams_cs": 0.2275, "dlon": dx, "dlat": dy, "x3max": ztop,
+            "s_time": s_time, "vset_fac": 0.85, "diam": diam, "rho": rng.uniform(800.0, 2600.0, nb),
+            "psi": rng.uniform(0.5, 1.0, nb), "cat": cat, "aggregate": agg}
+    return g, corner, mgrid, opts
+
+
+def c2p_2d_np(cc):
+    """4-corner average of a 2-D corner field (mass points)."""
+    return (cc[:-1, :-1] + cc[1:, :-1] + cc[:-1, 1:] + cc[1:, 1:]) / 4.0

@@ -163,6 +163,73 @@ int fall3d_ads_set_meteo(fall3d_ads_ctx *ctx, const void *u1, const void *u2, co
                          const void *w1, const void *w2, const void *k1, const void *k2, const void *k3,
                          const void *vs);
 
+/* ---- 2a. Meteo update on the device (F3D_update_meteo_term,
+ * Sources/mod_F3D.F90:564-969) --------------------------------------------
+ * From the meteo fields at the cell corners of the two meteo times that
+ * bound the interval (MY_MET%my_uc(...,its/ite) and friends) to the fields
+ * set_meteo takes, computed on the device and left there: ground corner
+ * winds zeroed (:662-669), corner -> face winds with one ghost face
+ * (Sources/mod_Grid.F90:685-752), their metric scaling (:761-870), the
+ * horizontal and vertical eddy diffusivities (Sources/mod_Phys.F90:406-653),
+ * the settling velocities of every bin (:673-960) and the scaling of k1, k2,
+ * k3 and v_s (Sources/mod_Grid.F90:880-1017). Diffusion and settling use the
+ * corner fields interpolated to s_time, (1 - s_time) its + s_time ite, as
+ * the reference does at the middle of the interval (mod_F3D.F90:704-705).
+ * Every value is formed in FP64; FP32 states store it rounded. Dry
+ * deposition velocities (phys_dry_deposition) are not added. Codes follow
+ * the reference's parameters. */
+typedef struct fall3d_meteo_options {
+  int map_h;    /* 0 Cartesian, 1 spherical (Sources/mod_Grid.F90:17-18) */
+  int map_v;    /* 0 Cartesian, 1 sigma without decay, 2 sigma with linear decay (:22-24) */
+  int modkh;    /* horizontal diffusion: 1 constant, 2 CMAQ, 4 RAMS (Sources/mod_Phys.F90:53-56) */
+  int modkv;    /* vertical diffusion: 1 constant, 3 similarity */
+  int modv;     /* settling model 1..7: Arastopour, Ganser, Wilson, Dellino, Pfeiffer,
+                 * Dioguardi 2017, Dioguardi 2018 (:44-50) */
+  int nbins;    /* must equal the context's */
+  double kh0, kv0;    /* constant diffusivities (MY_MOD%kh0, kv0) */
+  double rams_cs;     /* MY_MOD%RAMS_CS */
+  double dlon, dlat;  /* MY_GRID%dlon, dlat (CMAQ) */
+  double x3max;       /* MY_GRID%X3max (sigma with linear decay) */
+  double s_time;      /* interpolation factor of the diffusion and settling inputs */
+  double vset_fac;    /* MY_AGR%vset_fac, applied to aggregate bins */
+  const double *diam, *rho, *psi;  /* per bin: diameter, density, shape factor (MY_BIN) */
+  const int *cat;                  /* per bin: category (2 = aerosol: v_s = 0) */
+  const int *aggregate;            /* per bin: 1 for an aggregate bin */
+} fall3d_meteo_options;
+
+/* Grid arrays of the update (host arrays of real_kind). */
+typedef struct fall3d_meteo_grid {
+  const void *Hm1_p;           /* (1:ny) */
+  const void *Hm2_p;           /* (-1:ny+2) */
+  const void *Hm3_p;           /* (1:nx,1:ny) */
+  const void *dhdx_p, *dhdy_p; /* (1:nx,1:ny); read for map_v 1, 2 */
+  const void *z_c;             /* (1:nx+1,1:ny+1,1:nz+1) corner heights; read for similarity */
+  const void *h_c;             /* (1:nx+1,1:ny+1) corner topography; read for similarity */
+  const void *X3_c;            /* (1:nz+1); read for map_v 2 */
+  const void *X3_p;            /* (1:nz); read for map_v 2 */
+} fall3d_meteo_grid;
+
+/* Corner fields (host arrays of real_kind), [0] at the interval's first
+ * meteo time, [1] at its second: 3-D (1:nx+1,1:ny+1,1:nz+1), 2-D
+ * (1:nx+1,1:ny+1). tvc, pblhc, ustc, monc are read for similarity only. */
+typedef struct fall3d_meteo_corner {
+  const void *uc[2], *vc[2], *wc[2];
+  const void *tc[2], *tvc[2], *rhoc[2];
+  const void *pblhc[2], *ustc[2], *monc[2];
+} fall3d_meteo_corner;
+
+/* Runs the update and makes its output the context's meteo (as set_meteo);
+ * returns ESTATE when a settling velocity does not converge (as the
+ * reference stops with 'no convergence in vset'). */
+int fall3d_ads_update_meteo(fall3d_ads_ctx *ctx, const fall3d_meteo_corner *corner, const fall3d_meteo_grid *grid,
+                            const fall3d_meteo_options *opts);
+/* Reads the context's meteo back in the layouts set_meteo takes (any
+ * pointer may be NULL): u1/u2 (nx+3,ny,nz), v1/v2 (ny+3,nx,nz), w1/w2
+ * (nz+3,nx,ny), k1 (nx+4,ny,nz), k2 (ny+4,nx,nz), k3 (nz+4,nx,ny),
+ * vs (nx,ny,nz+3,nbins), real_kind. */
+int fall3d_ads_get_meteo(fall3d_ads_ctx *ctx, void *u1, void *u2, void *v1, void *v2, void *w1, void *w2, void *k1,
+                         void *k2, void *k3, void *vs);
+
 /* Dense source rate s(1:nx,1:ny,1:nz,nbins) (MY_TRA%my_s), injected before
  * the sweeps as c += dt s / (dX1_p dX2_p dX3_p Hm1 Hm2 Hm3)
  * (Sources/mod_F3D.F90:1456-1473); stored sparsely on the device (only
