@@ -50,6 +50,7 @@ EXPORTS = (
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
     "fall3d_ads_copy_probe", "fall3d_ads_check_state", "fall3d_ads_learned_fallbacks", "fall3d_ads_snapshot",
     "fall3d_ads_restore", "fall3d_ens_statistics", "fall3d_ads_update_meteo", "fall3d_ads_get_meteo",
+    "fall3d_ads_postprocess",
     "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
@@ -85,6 +86,13 @@ class _MeteoGrid(ctypes.Structure):
 
 class _MeteoCorner(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p * 2) for n in ("uc", "vc", "wc", "tc", "tvc", "rhoc", "pblhc", "ustc", "monc")]
+
+
+class _OutputArgs(ctypes.Structure):
+    _fields_ = [("Hm1_p", ctypes.c_void_p), ("Hm2_p", ctypes.c_void_p), ("Hm3_p", ctypes.c_void_p),
+                ("z_c", ctypes.c_void_p), ("bin_species", ctypes.c_void_p), ("species", ctypes.c_int),
+                ("species_category", ctypes.c_int), ("so2_du", ctypes.c_int), ("npm", ctypes.c_int),
+                ("pm_um", ctypes.c_void_p), ("bin_diam", ctypes.c_void_p)]
 
 
 METEO_MAP_H = {"cartesian": 0, "spherical": 1}
@@ -147,6 +155,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_learned_fallbacks": ([_vp, _vp], _int),
         "fall3d_ads_update_meteo": ([_vp, _vp, _vp, _vp], _int),
         "fall3d_ads_get_meteo": ([_vp] * 11, _int),
+        "fall3d_ads_postprocess": ([_vp] * 10, _int),
         "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
         "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
         "fall3d_ads_set_fluxes": ([_vp, _dp], _int),
@@ -450,6 +459,39 @@ class ADS:
                   "k3": (nz + 4, nx, ny), "vs": self.shape_vs}
         out = {k: np.zeros(v, dtype=self.dtype, order="F") for k, v in shapes.items()}
         self._chk(load_library().fall3d_ads_get_meteo(self._h, *(out[k].ctypes.data for k in shapes)), "get_meteo")
+        return out
+
+    def postprocess(self, z_c=None, hm1=None, hm2=None, hm3=None, bin_species=None, species=0, species_category=1,
+                    so2_du=False, pm_um=(), bin_diam=None, want=("con_bins", "con", "con_sur", "col_mass",
+                                                                  "col_mass_pm", "cloud_top", "grn_load",
+                                                                  "grn_load_bin")):
+        """nc_IO_out_res's result variables from the device state (see
+        fall3d_ads_postprocess); returns a dict of FP64 corner arrays."""
+        nx, ny, nz, nb = self.nx, self.ny, self.nz, self.nb
+        keep = []
+
+        def arr(x, shape, name):
+            if x is None:
+                return None
+            a = self._arr(x, shape, name)
+            keep.append(a)
+            return a.ctypes.data
+
+        spe = None if bin_species is None else np.ascontiguousarray(bin_species, dtype=np.int32)
+        pm = np.ascontiguousarray(pm_um, dtype=np.float64)
+        diam = None if bin_diam is None else np.ascontiguousarray(bin_diam, dtype=np.float64)
+        a = _OutputArgs(arr(hm1, (ny,), "Hm1_p"), arr(hm2, (ny + 4,), "Hm2_p"), arr(hm3, (nx, ny), "Hm3_p"),
+                        arr(z_c, (nx + 1, ny + 1, nz + 1), "z_c"), None if spe is None else spe.ctypes.data,
+                        int(species), int(species_category), int(bool(so2_du)), int(pm.size),
+                        pm.ctypes.data if pm.size else None, None if diam is None else diam.ctypes.data)
+        c2, c3 = (nx + 1, ny + 1), (nx + 1, ny + 1, nz + 1)
+        shapes = {"con_bins": c3 + (nb,), "con": c3, "con_sur": c2, "col_mass": c2,
+                  "col_mass_pm": c2 + (max(pm.size, 1),), "cloud_top": c2, "grn_load": c2, "grn_load_bin": c2 + (nb,)}
+        out = {k: np.zeros(shapes[k], order="F") for k in want}
+        ptr = lambda k: out[k].ctypes.data if k in out else None
+        self._chk(load_library().fall3d_ads_postprocess(self._h, ctypes.byref(a), *(ptr(k) for k in shapes)),
+                  "postprocess")
+        del keep
         return out
 
     def set_problem(self, prob):

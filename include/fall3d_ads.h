@@ -230,6 +230,40 @@ int fall3d_ads_update_meteo(fall3d_ads_ctx *ctx, const fall3d_meteo_corner *corn
 int fall3d_ads_get_meteo(fall3d_ads_ctx *ctx, void *u1, void *u2, void *v1, void *v2, void *w1, void *w2, void *k1,
                          void *k2, void *k3, void *vs);
 
+/* ---- 2b. Output post-processing (nc_IO_out_res, Sources/mod_nc_IO.F90:
+ * 3206-4335) -----------------------------------------------------------------
+ * From the context's state, on the device: c scaled back by the map
+ * factors, converted to g/m3 and clipped at 0 (:3397-3416), averaged to the
+ * cell corners (Sources/mod_Grid.F90:1150-1181, ghosts included); the ground
+ * accumulation averaged to the corners (:1190-1220, kg/m2); and for one
+ * species the reference's result variables: CON, CON_SUR, COL_MASS (g/m2, or
+ * Dobson units with so2_du: / 64 x 2.238e3), COL_MASS_PM (bins with diameter
+ * <= each threshold; none for an aerosol species), CLOUD_TOP (height of the
+ * highest corner above 50 ug/m3, else 0) and GRN_LOAD, plus CON_BIN and
+ * GRN_LOAD_BIN for every bin (mod_nc_IO.F90:3506-4020). Wet deposition is
+ * not part of the ground load here (no wet deposition runs on the device);
+ * the z and flight-level cuts, which interpolate these fields for the
+ * output files, are left to the caller. */
+typedef struct fall3d_output_args {
+  const void *Hm1_p, *Hm2_p, *Hm3_p; /* (1:ny), (-1:ny+2), (1:nx,1:ny) of real_kind; NULL = 1 */
+  const void *z_c;                   /* corner heights (1:nx+1,1:ny+1,1:nz+1), real_kind */
+  const int *bin_species;            /* species code of each bin; NULL = every bin one species */
+  int species;                       /* the species the species products sum */
+  int species_category;              /* its category (2 = aerosol: no PM loads) */
+  int so2_du;                        /* 1: COL_MASS in Dobson units */
+  int npm;                           /* number of PM thresholds (at most 8) */
+  const double *pm_um;               /* PM thresholds in micrometres */
+  const double *bin_diam;            /* diameter of each bin (m) */
+} fall3d_output_args;
+
+/* Outputs are FP64 host arrays at the corners, x fastest (any may be NULL):
+ * con_bins (nx+1,ny+1,nz+1,nbins), con (nx+1,ny+1,nz+1), con_sur, col_mass,
+ * cloud_top, grn_load (nx+1,ny+1), col_mass_pm (nx+1,ny+1,npm), grn_load_bin
+ * (nx+1,ny+1,nbins). */
+int fall3d_ads_postprocess(fall3d_ads_ctx *ctx, const fall3d_output_args *args, double *con_bins, double *con,
+                           double *con_sur, double *col_mass, double *col_mass_pm, double *cloud_top,
+                           double *grn_load, double *grn_load_bin);
+
 /* Dense source rate s(1:nx,1:ny,1:nz,nbins) (MY_TRA%my_s), injected before
  * the sweeps as c += dt s / (dX1_p dX2_p dX3_p Hm1 Hm2 Hm3)
  * (Sources/mod_F3D.F90:1456-1473); stored sparsely on the device (only

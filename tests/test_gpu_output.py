@@ -1,0 +1,60 @@
+"""fall3d_ads_postprocess (nc_IO_out_res's result variables) on the GPU
+against the CPU restatement (oracle/output_oracle.py) of the same state.
+
+The device reads the state it holds; the restatement reads that state as
+get_state returns it (FP64, or FP32 widened exactly, hi + lo for the
+compensated state, which is the sum the device forms). Both evaluate the
+same statements in the same order, so every output must be equal bit for
+bit, for the three states, with map factors, terrain-following corner
+heights, two species, PM thresholds, an aerosol species (no PM loads) and
+Dobson units.
+"""
+import numpy as np
+import pytest
+
+from fall3d_amd import ads
+from fall3d_amd.output import budget_log, result_variables
+from fall3d_amd.problem import operational_problem
+from oracle import output_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _zc(nx, ny, nz, seed):
+    rng = np.random.default_rng(seed)
+    h = rng.uniform(0.0, 800.0, (nx + 1, ny + 1))
+    lev = np.cumsum(np.concatenate([[0.0], rng.uniform(200.0, 400.0, nz)]))
+    top = lev[-1]
+    return np.asfortranarray(h[:, :, None] + lev[None, None, :] * (1.0 - h[:, :, None] / (2 * top)))
+
+
+@pytest.mark.parametrize("state", ["fp32", "fp32x2", "fp64"])
+def test_postprocess_matches_restatement(state):
+    print(f"case postprocess {state}", flush=True)
+    p = operational_problem(48, 36, 16, 5, seed=3)
+    nx, ny, nz, nb = p.nx, p.ny, p.nz, p.nb
+    a = ads.ADS(nx, ny, nz, nb, "minmod", "rk", 8, state)
+    a.set_problem(p)
+    dt = a.time_step(0.0, 2, 0.9)
+    a.run(dt, 12, 0)
+    c, acum = a.get_state()
+    rng = np.random.default_rng(7)
+    hm1, hm2, hm3 = rng.uniform(0.7, 1.0, ny), rng.uniform(0.9, 1.1, ny + 4), rng.uniform(0.85, 1.0, (nx, ny))
+    zc = _zc(nx, ny, nz, 5)
+    spe = np.array([0, 0, 1, 0, 1], dtype=np.int32)
+    diam = np.array([1e-6, 4e-6, 9e-6, 2e-5, 1e-4])
+    cases = [dict(species=0, pm_um=(2.5, 10.0)), dict(species=1, so2_du=True, pm_um=(2.5,)),
+             dict(species=0, species_category=2, pm_um=(10.0,))]
+    for kw in cases:
+        got = a.postprocess(z_c=zc, hm1=hm1, hm2=hm2, hm3=hm3, bin_species=spe, bin_diam=diam, **kw)
+        ref = O.postprocess(c, acum, z_c=zc, hm1=hm1, hm2=hm2, hm3=hm3, bin_species=spe, bin_diam=diam, **kw)
+        for k in got:
+            diff = np.count_nonzero(got[k] != ref[k])
+            if diff:
+                print(f"  {k}: {diff} differ, max {np.abs(got[k] - ref[k]).max():.3e}", flush=True)
+            assert diff == 0, (kw, k)
+        assert ref["cloud_top"].max() > 0.0 and ref["col_mass"].max() > 0.0
+    names = result_variables(a, "tephra", z_c=zc)
+    assert "tephra_col_mass" in names and names["tephra_grn_load"].shape == (nx + 1, ny + 1)
+    print(budget_log(a.budget(), 12, dt, 12 * dt, 0.0, 100 * dt, "t", a.budget()["injected"]), flush=True)
+    a.release()
