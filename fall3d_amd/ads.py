@@ -34,6 +34,7 @@ STATE_FP32, STATE_FP32X2, STATE_FP64 = 0, 1, 2
 LIMITERS = {"minmod": MINMOD, "superbee": SUPERBEE, "ospre": OSPRE}
 TIME_MARCHING = {"euler": EULER, "rk": RK, "runge-kutta": RK}
 STATES = {"fp32": STATE_FP32, "fp32x2": STATE_FP32X2, "fp64": STATE_FP64}
+BOUNDARIES = {"freeflow": 0, "dirichlet": 1, "periodic": 2}
 
 BUDGET_MASS_VOLUME, BUDGET_MASS_CELLS, BUDGET_GROUND, BUDGET_LATERAL, BUDGET_INJECTED, BUDGET_LEDGER = 0, 1, 2, 3, 4, 5
 BUDGET_LEN = 12
@@ -50,7 +51,7 @@ EXPORTS = (
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
     "fall3d_ads_copy_probe", "fall3d_ads_check_state", "fall3d_ads_learned_fallbacks", "fall3d_ads_snapshot",
     "fall3d_ads_restore", "fall3d_ens_statistics", "fall3d_ads_update_meteo", "fall3d_ads_get_meteo",
-    "fall3d_ads_postprocess",
+    "fall3d_ads_postprocess", "fall3d_ads_set_boundary",
     "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
@@ -156,6 +157,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_update_meteo": ([_vp, _vp, _vp, _vp], _int),
         "fall3d_ads_get_meteo": ([_vp] * 11, _int),
         "fall3d_ads_postprocess": ([_vp] * 10, _int),
+        "fall3d_ads_set_boundary": ([_vp, _int], _int),
         "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
         "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
         "fall3d_ads_set_fluxes": ([_vp, _dp], _int),
@@ -493,6 +495,12 @@ class ADS:
                   "postprocess")
         del keep
         return out
+
+    def set_boundary(self, mode="freeflow"):
+        """Ghost rule of the sweeps: "freeflow" (default), "dirichlet" or
+        "periodic" (the reference's zero-gradient copy)."""
+        m = BOUNDARIES[mode] if isinstance(mode, str) else int(mode)
+        self._chk(load_library().fall3d_ads_set_boundary(self._h, m), "set_boundary")
 
     def set_problem(self, prob):
         """Grid, meteo, source and initial state of a fall3d_amd.problem.Problem."""

@@ -236,6 +236,7 @@ void make_sweep_args(fall3d_ads_ctx *c, int dir, double dt, double stime, int wr
   a.nbin = t.bpg;
   a.nbins = c->nb;
   a.tm = c->tm;
+  a.bc = c->bc;
   a.write_ghosts = write_ghosts;
   a.dt = (float)dt;
   a.stime = (float)stime;
@@ -322,6 +323,7 @@ int run_sweep64(fall3d_ads_ctx *c, int dir, double dt, double stime, const doubl
   a.nb = c->nb;
   a.n = axis_n(c, dir);
   a.tm = c->tm;
+  a.bc = c->bc;
   a.nlines = axis_lines(c, dir);
   a.dt = dt;
   a.stime = stime;
@@ -1188,6 +1190,15 @@ int fall3d_ads_copy_probe(const void *src, void *dst, size_t n, void *stream) {
   hipStream_t st = (hipStream_t)stream;
   if (copy_probe((const float *)src, (float *)dst, n, st) != hipSuccess) return FALL3D_ADS_EHIP;
   return hipStreamSynchronize(st) == hipSuccess ? FALL3D_ADS_OK : FALL3D_ADS_EHIP;
+}
+
+int fall3d_ads_set_boundary(fall3d_ads_ctx *c, int mode) {
+  if (!c) return FALL3D_ADS_EINVAL;
+  if (mode < FALL3D_ADS_BC_FREEFLOW || mode > FALL3D_ADS_BC_PERIODIC)
+    return fail(c, FALL3D_ADS_EINVAL, "invalid boundary condition %d", mode);
+  c->bc = mode;
+  c->version++;  // captured steps hold the old rule in their arguments
+  return FALL3D_ADS_OK;
 }
 
 int fall3d_ads_set_graph(fall3d_ads_ctx *c, int on) {

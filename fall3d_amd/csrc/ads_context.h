@@ -47,6 +47,7 @@ struct PendingEvent {
 struct fall3d_ads_ctx {
   int nx = 0, ny = 0, nz = 0, nb = 0;
   int limiter = 1, tm = 2, kind = 8, comp = 0;
+  int bc = 0;  // boundary rule of the sweeps (FALL3D_ADS_BC_*)
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string err = "no error";

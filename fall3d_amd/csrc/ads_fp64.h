@@ -15,6 +15,7 @@ struct Sweep64Args {
   int nx, ny, nz, nb;
   int n;                      // cells along the sweep axis
   int tm;                     // 1 Euler, 2 RK4
+  int bc;                     // ghost rule: 0 free flow, 1 Dirichlet (bvalue 0), 2 zero-gradient
   long nlines;                // lines per bin
   double dt, stime;
   const double *vel1, *vel2;  // face winds along the axis; (1-stime) vel1 + stime vel2

@@ -150,12 +150,15 @@ __global__ void __launch_bounds__(256) k_sweep64(Sweep64Args a) {
     };
     auto kval = [&](int p) -> double { return a.kd[kb + (size_t)(p + 1) * vst]; };
 
-    // free-flow ghosts: inflow at a side (u(1) > 0, u(n+1) < 0) gives
-    // 2 bvalue - c with bvalue = 0, outflow a zero-gradient copy; the outer
-    // ghost copies the inner one
+    // boundary ghosts: with free flow, inflow at a side (u(1) > 0,
+    // u(n+1) < 0) gives 2 bvalue - c with bvalue = 0 and outflow a
+    // zero-gradient copy; Dirichlet always gives the former, the
+    // zero-gradient rule always the latter; the outer ghost copies the inner
     const double uend = wind(n + 1);
     const double c1 = line[cs];
-    const double gw = wind(1) > 0.0 ? 2.0 * 0.0 - c1 : c1;
+    const bool inw = a.bc == 0 ? (wind(1) > 0.0) : (a.bc == 1);
+    const bool ine = a.bc == 0 ? (uend < 0.0) : (a.bc == 1);
+    const double gw = inw ? 2.0 * 0.0 - c1 : c1;
     line[0] = gw;
     line[-cs] = gw;
 
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(256) k_sweep64(Sweep64Args a) {
         } else if (p <= n) {
           v = line[p * cs];
         } else if (p == n + 1) {
-          v = uend < 0.0 ? 2.0 * 0.0 - lr[1] : lr[1];
+          v = ine ? 2.0 * 0.0 - lr[1] : lr[1];
           line[p * cs] = v;
         } else if (p == n + 2) {
           v = lr[1];

@@ -34,6 +34,7 @@ struct SweepArgs {
   int nbins;              // total bins
   int tm;                 // 1 Euler, 2 RK4
   int write_ghosts;       // store the free-flow ghosts back into c
+  int bc;                 // ghost rule: 0 free flow, 1 Dirichlet (bvalue 0), 2 zero-gradient ("periodic")
   float dt, stime;
   const float *vel1, *vel2;  // face winds along the sweep axis (device layout)
   const float *kd;           // diffusion at mass points along the sweep axis

@@ -334,8 +334,10 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       }
       const float c1 = bload(rc[j], vc, cpos(1));
       const float cnn = bload(rc[j], vc, cpos(n));
-      gL[j] = (vfirst > 0.0f) ? (2.0f * 0.0f - c1) : c1;
-      gR[j] = (vlast < 0.0f) ? (2.0f * 0.0f - cnn) : cnn;
+      const bool inL = a.bc == 0 ? (vfirst > 0.0f) : (a.bc == 1);
+      const bool inR = a.bc == 0 ? (vlast < 0.0f) : (a.bc == 1);
+      gL[j] = inL ? (2.0f * 0.0f - c1) : c1;
+      gR[j] = inR ? (2.0f * 0.0f - cnn) : cnn;
       const uint32_t vg = vstore[j];  // always written: the sweep reads them back
       bstore(rc[j], vg, cpos(-1), gL[j]);
       bstore(rc[j], vg, cpos(0), gL[j]);
@@ -896,12 +898,12 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       __syncthreads();
       if (xl) cleft = xch[wave - 1][5];
     }
-    if (first) gL = (vbeg > 0.0f) ? (2.0f * 0.0f - cn[0]) : cn[0];
+    if (first) gL = (a.bc == 0 ? (vbeg > 0.0f) : (a.bc == 1)) ? (2.0f * 0.0f - cn[0]) : cn[0];
     if (last) {
       float cl = (qn < 0) ? cleft : cn[0];
 #pragma unroll
       for (int q = 1; q < C; ++q) cl = (q == qn) ? cn[q] : cl;
-      gR = (vend < 0.0f) ? (2.0f * 0.0f - cl) : cl;
+      gR = (a.bc == 0 ? (vend < 0.0f) : (a.bc == 1)) ? (2.0f * 0.0f - cl) : cl;
 #pragma unroll
       for (int q = 0; q < C; ++q) {
         cn[q] = (q > qn) ? gR : cn[q];

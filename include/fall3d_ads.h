@@ -163,6 +163,17 @@ int fall3d_ads_set_meteo(fall3d_ads_ctx *ctx, const void *u1, const void *u2, co
                          const void *w1, const void *w2, const void *k1, const void *k2, const void *k3,
                          const void *vs);
 
+/* Boundary rule of the sweeps' ghost cells on every physical side.
+ * ADS_initialize hard-wires free flow (Sources/mod_ADS.F90:116); the
+ * reference also holds dirichlet (:1330-1399: ghost = 2 bvalue - c, bvalue
+ * 0) and periodic (:1403-1448: in the reference a zero-gradient copy of the
+ * first cell, not a wrap-around) behind the same procedure pointer. The
+ * outer ghost copies the inner one in every rule. Default FREEFLOW. */
+#define FALL3D_ADS_BC_FREEFLOW 0
+#define FALL3D_ADS_BC_DIRICHLET 1
+#define FALL3D_ADS_BC_PERIODIC 2
+int fall3d_ads_set_boundary(fall3d_ads_ctx *ctx, int mode);
+
 /* ---- 2a. Meteo update on the device (F3D_update_meteo_term,
  * Sources/mod_F3D.F90:564-969) --------------------------------------------
  * From the meteo fields at the cell corners of the two meteo times that

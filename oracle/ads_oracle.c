@@ -60,7 +60,10 @@ enum { TM_EULER = 1, TM_RK = 2 };
 enum {
   CFG_NX = 0, CFG_NY, CFG_NZ, CFG_NB,
   CFG_BW, CFG_BE, CFG_BS, CFG_BN, CFG_BD, CFG_BU,
-  CFG_LIMITER, CFG_TM, CFG_LEN
+  CFG_LIMITER, CFG_TM,
+  CFG_BCMODE, /* 0 freeflow (the reference's choice, mod_ADS.F90:116), 1 dirichlet (:1330-1399),
+                 2 periodic (:1403-1448, in the reference a zero-gradient copy) */
+  CFG_LEN
 };
 
 /* ---------------------------------------------------------------------- */
@@ -211,18 +214,23 @@ static void integrate_line(int lim, int tm, int n, double dt, double *line,
   ledger[1] = g1;
 }
 
-/* Free-flow ghosts for one gathered line, Sources/mod_ADS.F90:1239-1324:
- * inflow (u_first > 0 / u_last < 0) gives 2*bvalue - c with bvalue = 0
- * (:117), otherwise a zero-gradient copy; the outer ghost copies the inner. */
+/* Boundary ghosts for one gathered line. Free flow (mode 0,
+ * Sources/mod_ADS.F90:1239-1324): inflow (u_first > 0 / u_last < 0) gives
+ * 2*bvalue - c with bvalue = 0 (:117), otherwise a zero-gradient copy.
+ * Dirichlet (mode 1, :1330-1399): 2*bvalue - c on every physical side.
+ * Periodic (mode 2, :1403-1448): a zero-gradient copy on every side. The
+ * outer ghost copies the inner one. */
 static void freeflow_line(int n, double *line, double ubeg, double uend,
-                          int bbeg, int bend) {
+                          int bbeg, int bend, int mode) {
   if (bbeg) {
-    if (ubeg > 0.0) line[0] = 2.0 * 0.0 - line[1];
+    const int in = mode == 0 ? (ubeg > 0.0) : (mode == 1);
+    if (in) line[0] = 2.0 * 0.0 - line[1];
     else line[0] = line[1];
     line[-1] = line[0];
   }
   if (bend) {
-    if (uend < 0.0) line[n + 1] = 2.0 * 0.0 - line[n];
+    const int in = mode == 0 ? (uend < 0.0) : (mode == 1);
+    if (in) line[n + 1] = 2.0 * 0.0 - line[n];
     else line[n + 1] = line[n];
     line[n + 2] = line[n + 1];
   }
@@ -295,7 +303,7 @@ static void sweep(int dir, const int *cfg, const double *const *grid,
         }
         /* gather (ghosts included), apply BC, integrate, scatter */
         for (int i = -1; i <= n + 2; ++i) line[i] = c[(ptrdiff_t)base + (ptrdiff_t)i * (ptrdiff_t)stride];
-        freeflow_line(n, line, uu[1], uu[n + 1], bbeg, bend);
+        freeflow_line(n, line, uu[1], uu[n + 1], bbeg, bend, cfg[CFG_BCMODE]);
         double fr[2], lg[2];
         integrate_line(lim, tm, n, dt, line, dxp, dxb, uu, kk, &w, fr, lg);
         for (int i = -1; i <= n + 2; ++i) c[(ptrdiff_t)base + (ptrdiff_t)i * (ptrdiff_t)stride] = line[i];

@@ -24,6 +24,7 @@ _lib = None
 
 LIMITERS = {"minmod": 1, "superbee": 2, "ospre": 3}
 TIME_MARCHING = {"euler": 1, "rk": 2}
+BC_MODES = {"freeflow": 0, "dirichlet": 1, "periodic": 2}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -80,11 +81,12 @@ def _p(a: np.ndarray):
 class Oracle:
     """FP64 oracle bound to one problem geometry (a `fall3d_amd.problem.Problem`)."""
 
-    def __init__(self, prob, limiter="minmod", time_marching="rk", boundaries=(1, 1, 1, 1, 1, 1)):
+    def __init__(self, prob, limiter="minmod", time_marching="rk", boundaries=(1, 1, 1, 1, 1, 1), bc="freeflow"):
         self.prob = prob
         cfg = [prob.nx, prob.ny, prob.nz, prob.nb, *boundaries,
                LIMITERS[limiter] if isinstance(limiter, str) else int(limiter),
-               TIME_MARCHING[time_marching] if isinstance(time_marching, str) else int(time_marching)]
+               TIME_MARCHING[time_marching] if isinstance(time_marching, str) else int(time_marching),
+               BC_MODES[bc] if isinstance(bc, str) else int(bc)]
         self._cfg = (ctypes.c_int * len(cfg))(*cfg)
         g = prob.grid
         self._gridarrs = [np.ascontiguousarray(g[k], dtype=np.float64)

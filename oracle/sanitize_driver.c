@@ -52,7 +52,7 @@ static double *fill(size_t n, double lo, double hi) {
 }
 
 static int run_problem(int nx, int ny, int nz, int nb, int lim, int tm, double *checksum) {
-  int cfg[12] = {nx, ny, nz, nb, 1, 1, 1, 1, 1, 1, lim, tm};
+  int cfg[13] = {nx, ny, nz, nb, 1, 1, 1, 1, 1, 1, lim, tm, 0};
   double *g[6];
   g[0] = fill((size_t)nx + 4, 800.0, 1200.0);
   g[1] = fill((size_t)nx + 3, 800.0, 1200.0);

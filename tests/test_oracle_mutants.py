@@ -18,16 +18,21 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "ads_oracle.c")
 PINS = [os.path.join(ROOT, "tests", "test_oracle_pins.py"),
-        os.path.join(ROOT, "tests", "test_oracle_pins_boundaries.py")]
+        os.path.join(ROOT, "tests", "test_oracle_pins_boundaries.py"),
+        os.path.join(ROOT, "tests", "test_oracle_pins_bcmodes.py")]
 
 # (name, text in the oracle, planted replacement, occurrences to replace)
 MUTANTS = [
     ("inflow ghost copies the first cell",
-     "if (ubeg > 0.0) line[0] = 2.0 * 0.0 - line[1];", "if (ubeg > 0.0) line[0] = line[1];", 1),
+     "if (in) line[0] = 2.0 * 0.0 - line[1];", "if (in) line[0] = line[1];", 1),
     ("inflow test not strict",
-     "if (ubeg > 0.0) line[0]", "if (ubeg >= 0.0) line[0]", 1),
+     "mode == 0 ? (ubeg > 0.0)", "mode == 0 ? (ubeg >= 0.0)", 1),
     ("last-face inflow sign flipped",
-     "if (uend < 0.0) line[n + 1]", "if (uend > 0.0) line[n + 1]", 1),
+     "mode == 0 ? (uend < 0.0)", "mode == 0 ? (uend > 0.0)", 1),
+    ("dirichlet ghosts follow the velocity",
+     "mode == 0 ? (ubeg > 0.0) : (mode == 1)", "mode <= 1 ? (ubeg > 0.0) : 0", 1),
+    ("zero-gradient rule reflects",
+     "mode == 0 ? (uend < 0.0) : (mode == 1)", "mode == 0 ? (uend < 0.0) : (mode >= 1)", 1),
     ("second ghost not a copy of the first",
      "    line[-1] = line[0];\n", "    line[-1] = line[1];\n", 1),
     ("reference accumulator with F - P",
