@@ -132,20 +132,24 @@ __device__ __forceinline__ void block_partials5(double v0, double v1, double v2,
   }
 }
 
-// FP32-state final update with an exact FP64 residual: the cell's new value
-// c + F (gq - gq1) is formed in FP64 (F the FP64 final factor), stored
-// rounded to FP32, and what the rounding dropped, times the cell's weight w,
-// is added to res. Interior faces then cancel between their two cells to
-// FP64 rounding, whatever the spacing.
+// FP32-state final update with an FP64 rounding residual: the cell's new
+// value c + F (gq - gq1) is formed in FP64 (F the FP64 final factor) and
+// stored rounded to FP32; what the rounding dropped, exact in FP64, is
+// scaled by the cell's weight w and summed into res. The sum runs in FP32:
+// its error is ~2^-24 of the residual, itself ~2^-24 of the state, so the
+// budget still closes to FP64 rounding, and FP32 keeps the accumulation off
+// the FP64 pipe of the VALU-bound sweeps. Interior faces cancel between their
+// two cells to FP64 rounding, whatever the spacing.
+// fence: close the accumulation here. Without it the compiler defers the
+// residual arithmetic of a whole unrolled block to its end and keeps every
+// FP64 update alive until then (register spills); the streaming sweeps fence
+// every other iteration, which leaves the scheduler room between fences.
 __device__ __forceinline__ float final_update_fp64(float c, float gq, float gq1, double F, float w, bool count,
-                                                   double &res) {
+                                                   float &res, bool fence = true) {
   const double c64 = fma((double)gq - (double)gq1, F, (double)c);
   const float cn = (float)c64;
-  res = fma(c64 - (double)cn, count ? (double)w : 0.0, res);
-  // the accumulator is final here: without this the compiler defers the
-  // residual arithmetic of a whole unrolled block to its end and keeps every
-  // FP64 update alive until then (register spills)
-  __asm__ volatile("" : "+v"(res));
+  res = fmaf((float)(c64 - (double)cn), count ? w : 0.0f, res);
+  if (fence) __asm__ volatile("" : "+v"(res));
   return cn;
 }
 
@@ -352,9 +356,9 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
     }
   }
 
-  double res[BPT];  // FP32 state: rounding residual of this lane's final updates, per bin
+  float res[BPT];  // FP32 state: rounding residual of this lane's final updates, per bin
 #pragma unroll
-  for (int j = 0; j < BPT; ++j) res[j] = 0.0;
+  for (int j = 0; j < BPT; ++j) res[j] = 0.0f;
   // rings: cells (16) and low words (8, fetched kPF ahead of the final
   // update) per bin; raw face data (8) and face coefficients (8). The wind
   // lerp and K are shared by the bins of a thread; in the z sweep the
@@ -600,7 +604,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
           bstore(rl[j], vo, cpos(q), lv);
         } else {
           (void)vo;
-          out_f[(j * 4 + (u & 3)) * 64 + lane] = final_update_fp64(cn[j][cl], gq, gq1, fq, row[5], q >= 1, res[j]);
+          out_f[(j * 4 + (u & 3)) * 64 + lane] = final_update_fp64(cn[j][cl], gq, gq1, fq, row[5], q >= 1, res[j], (u & 1) == 1);
         }
         if constexpr (FOUT) {  // face mass flux per unit area: faces q (1..n) and n+1
           const uint32_t fo = fpos(q), fo1 = fpos(q + 1);
@@ -684,7 +688,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
       re += (double)RFe * area * dt;
       lb -= (double)tl1 * f1 * a.dxb64[1] * area;
       le += (double)trn * fn * a.dxb64[n] * area;
-      rr += res[j] * area;
+      rr += (double)res[j] * area;
       if constexpr (DIR == 2) {
         const size_t ai = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(t2 + 1) + (size_t)(ny + 4) * bins[j]);
         a.acum[ai] = a.acum[ai] + fabs((double)RFb) * dt;
@@ -1031,7 +1035,7 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
     // state; for the FP32 state the raw face values, scaled in FP64 below
     float tl0 = 0.0f, trn = 0.0f;
     double fl0 = 1.0, fln = 1.0;
-    double res = 0.0;
+    float res = 0.0f;
     float hv[C], lv[C];
 #pragma unroll
     for (int q = 0; q < C; ++q) {
@@ -1079,7 +1083,7 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
       if (qn >= 0 && qn <= C - 1) {  // holds cell n
         acc_le += (double)trn * fln * a.dxb64[n] * area;
       }
-      acc_res += res * area;
+      acc_res += (double)res * area;
     }
   }
   block_partials5(acc_rb, acc_re, acc_lb, acc_le, acc_res, a.partials, a.rpartials, (size_t)by * a.gx + bx);
