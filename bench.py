@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB with torch and with the library's "
                          "16-byte buffer-load copy (calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--hbm-reference", type=int, default=0, metavar="MIB",
+                    help="only time torch's streaming kernels (add, scale, copy, sum) on MIB MiB arrays, "
+                         "as a reference for the bandwidth HBM delivers to plain streams, and exit")
     return ap.parse_args()
 
 
@@ -73,6 +76,37 @@ def log(msg):
 
 
 # ---------------------------------------------------------------------------
+def hbm_reference(mib, reps=20):
+    """Bandwidth of torch's elementwise streams on MIB MiB FP32 arrays, bytes
+    read + written per call / average call time (torch events around reps
+    calls after 3 untimed ones): the rate HBM delivers to a plain stream."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda:0")
+    b = torch.rand(n, device="cuda:0")
+    c = torch.empty_like(a)
+
+    def rate(f, nbytes):
+        for _ in range(3):
+            f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return round(nbytes / (e0.elapsed_time(e1) * 1e-3 / reps) / 1e9, 1)
+
+    out = {"hbm_reference_mib": mib, "unit": "GB/s (bytes read + written per call / time)",
+           "add": rate(lambda: torch.add(a, b, out=c), 3 * 4 * n),
+           "scale": rate(lambda: torch.mul(a, 2.0, out=c), 2 * 4 * n),
+           "copy": rate(lambda: c.copy_(a), 2 * 4 * n),
+           "sum": rate(lambda: a.sum(), 4 * n)}
+    print(json.dumps(out))
+
+
 def copy_probe(mib):
     """Known-traffic references for the HBM counters: each copy reads and
     writes exactly mib MiB (torch's copy kernel, and the library's copy with
@@ -415,6 +449,9 @@ def main():
         sys.exit(2)
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+        return
+    if args.hbm_reference > 0:
+        hbm_reference(args.hbm_reference)
         return
     if args.workload == "ensemble":
         out = ensemble(args)

@@ -72,10 +72,17 @@ hipError_t finalize(double *acc, const ReduceJob &jobs, hipStream_t st);
 hipError_t finalize_batch(const BatchJob *jobs, int njobs, hipStream_t st);
 hipError_t source_inject_batch(const SourceMember *tab, const int *bstart, int nmem, int nblocks, hipStream_t st);
 hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *out, hipStream_t st);
-// state: FP64 c64 when given, else chi (+ clo)
+// FP64 spacings of the two mass measures: p1..p3 the mass-point spacings dX_p
+// (position p at index p+1), b1..b3 the cell sizes dX_b (index p)
+struct MassSpacings {
+  const double *p1, *p2, *p3;
+  const double *b1, *b2, *b3;
+};
+// state: FP64 c64 when given, else chi (+ clo). out[0] = sum of c dXp dYp dZp
+// (mass_volume), out[1] = sum of c dXb dYb dZb over the interior; part holds
+// part_cap doubles of per-workgroup sums
 hipError_t mass(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,
-                const double *d1, const double *d2, const double *d3, int off, double *part, int part_cap,
-                double *out, hipStream_t st);
+                const MassSpacings &sp, double *part, int part_cap, double *out, hipStream_t st);
 hipError_t solve_1d(const Solve1DArgs &a, hipStream_t st);
 // non-finite cells, negative cells, smallest finite value of the interior -> out[0..2]
 hipError_t health(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,

@@ -4,6 +4,8 @@
 // single-line ADS_solve_1D.
 #include "ads_aux.h"
 
+#include <climits>
+
 namespace f3d {
 
 // ---------------------------------------------------------------------------
@@ -438,37 +440,40 @@ __device__ __forceinline__ double wmax(double v) {
 
 __global__ void k_cfl(CflArgs a, double *__restrict__ part) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const size_t ncell = (size_t)nx * ny * nz;
+  // one wave per (j, k) row at a time, lanes along x
+  const int nrows = ny * nz;
+  const int nw = gridDim.x * (blockDim.x >> 6);
   double m1 = 0.0, m2 = 0.0, m3 = 0.0;
   const double s = a.stime, s1 = 1.0 - a.stime;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx) + 1;
-    const int j = (int)((c / nx) % ny) + 1;
-    const int k = (int)(c / ((size_t)nx * ny)) + 1;
-    const double dx = a.dx1p[i + 1], dy = a.dx2p[j + 1], dz = a.dx3p[k + 1];
-    const size_t iu = (size_t)i + (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * (k - 1));
-    const size_t iv = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * (k - 1));
-    const size_t iw = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * k);
-    const double u = s1 * (double)a.u1[iu] + s * (double)a.u2[iu];
-    const double v = s1 * (double)a.v1[iv] + s * (double)a.v2[iv];
-    const double w = s1 * (double)a.w1[iw] + s * (double)a.w2[iw];
-    const double k1 = a.k1[(size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j - 1) + (size_t)ny * (k - 1))];
-    const double k2 = a.k2[(size_t)(i - 1) + (size_t)nx * ((size_t)(j + 1) + (size_t)(ny + 4) * (k - 1))];
-    const double k3 = a.k3[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k + 1))];
-    double wm = 0.0;
-    for (int b = 0; b < a.nb; ++b) {
-      const double vs = a.vs[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)k + (size_t)(nz + 3) * b))];
-      wm = fmax(wm, fabs(w - vs));
-    }
-    const double t1 = 2.0 * k1 / (dx * dx) + fabs(u) / dx;
-    const double t2 = 2.0 * k2 / (dy * dy) + fabs(v) / dy;
-    const double t3 = 2.0 * k3 / (dz * dz) + wm / dz;
-    if (a.criterion == 1) {
-      m1 = fmax(m1, t1);
-      m2 = fmax(m2, t2);
-      m3 = fmax(m3, t3);
-    } else {
-      m1 = fmax(m1, t1 + t2 + t3);
+  for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < nrows; r += nw) {
+    const int k = r / ny + 1;
+    const int j = r - (k - 1) * ny + 1;
+    for (int i = (threadIdx.x & 63) + 1; i <= nx; i += 64) {
+      const double dx = a.dx1p[i + 1], dy = a.dx2p[j + 1], dz = a.dx3p[k + 1];
+      const size_t iu = (size_t)i + (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * (k - 1));
+      const size_t iv = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * (k - 1));
+      const size_t iw = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * k);
+      const double u = s1 * (double)a.u1[iu] + s * (double)a.u2[iu];
+      const double v = s1 * (double)a.v1[iv] + s * (double)a.v2[iv];
+      const double w = s1 * (double)a.w1[iw] + s * (double)a.w2[iw];
+      const double k1 = a.k1[(size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j - 1) + (size_t)ny * (k - 1))];
+      const double k2 = a.k2[(size_t)(i - 1) + (size_t)nx * ((size_t)(j + 1) + (size_t)(ny + 4) * (k - 1))];
+      const double k3 = a.k3[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k + 1))];
+      double wm = 0.0;
+      for (int b = 0; b < a.nb; ++b) {
+        const double vs = a.vs[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)k + (size_t)(nz + 3) * b))];
+        wm = fmax(wm, fabs(w - vs));
+      }
+      const double t1 = 2.0 * k1 / (dx * dx) + fabs(u) / dx;
+      const double t2 = 2.0 * k2 / (dy * dy) + fabs(v) / dy;
+      const double t3 = 2.0 * k3 / (dz * dz) + wm / dz;
+      if (a.criterion == 1) {
+        m1 = fmax(m1, t1);
+        m2 = fmax(m2, t2);
+        m3 = fmax(m3, t3);
+      } else {
+        m1 = fmax(m1, t1 + t2 + t3);
+      }
     }
   }
   __shared__ double red[3][16];
@@ -505,8 +510,9 @@ __global__ void k_cfl_final(const double *__restrict__ part, int nb, double cfl,
 
 hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *out, hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
-  const size_t ncell = (size_t)a.nx * a.ny * a.nz;
-  int nb = (int)((ncell + 255) / 256);
+  const size_t rows = (size_t)a.ny * a.nz;  // one wave per row, 4 waves per workgroup
+  if (rows > (size_t)INT_MAX) return hipErrorInvalidValue;
+  int nb = (int)((rows + 3) / 4);
   if (nb > part_cap) nb = part_cap;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, a, part);
@@ -518,56 +524,104 @@ hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *o
 // mass in the volume, Sources/mod_Grid.F90:1364-1375 (FP64)
 // ---------------------------------------------------------------------------
 
-__global__ void k_mass(const float *__restrict__ chi, const float *__restrict__ clo, const double *__restrict__ c64,
-                       int nx, int ny, int nz, int nb, const double *__restrict__ d1, const double *__restrict__ d2,
-                       const double *__restrict__ d3, int off, double *__restrict__ part) {
-  const size_t ncell = (size_t)nx * ny * nz * nb;
-  double m = 0.0;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx) + 1;
-    size_t r = c / nx;
-    const int j = (int)(r % ny) + 1;
-    r /= ny;
-    const int k = (int)(r % nz) + 1;
-    const int b = (int)(r / nz);
-    const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
-    double cv;
-    if (c64) {
-      cv = c64[ci];
-    } else {
-      cv = (double)chi[ci];
-      if (clo) cv += (double)clo[ci];
-    }
-    m += cv * d1[i + off] * d2[j + off] * d3[k + off];
-  }
-  block_sum_store(m, part);
+// Interior rows (j, k, bin) of the state, one wave per row at a time and the
+// lanes along x: the row's position comes from one 32-bit division per row,
+// and its cells are read coalesced.
+struct RowWalk {
+  int j, k, b;    // 1-based j, k; bin
+  size_t base;    // element offset of cell (0, j, k) of bin b
+};
+__device__ __forceinline__ RowWalk row_of(int r, int nx, int ny, int nz) {
+  RowWalk w;
+  const int t = r / ny;
+  w.j = r - t * ny + 1;
+  w.k = t % nz + 1;
+  w.b = t / nz;
+  w.base = (size_t)1 + (size_t)(nx + 4) * ((size_t)(w.j + 1) + (size_t)(ny + 4) * ((size_t)(w.k + 1) + (size_t)(nz + 4) * w.b));
+  return w;
+}
+// state value of element ci: MODE 0 FP32 (chi), 1 compensated (chi + clo), 2 FP64 (c64)
+template <int MODE>
+__device__ __forceinline__ double state_at(const float *chi, const float *clo, const double *c64, size_t ci) {
+  if constexpr (MODE == 2) return c64[ci];
+  double v = (double)chi[ci];
+  if constexpr (MODE == 1) v += (double)clo[ci];
+  return v;
 }
 
-__global__ void k_sum_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
-  __shared__ double red[16];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  s = wsum(s);
-  if (lane == 0) red[wid] = s;
+// The two mass measures of the budget in one pass over the state: with the
+// mass-point spacings dX_p (the reference's mass_volume; position p at index
+// p+1) and with the cell sizes dX_b (index p). Per cell the products are
+// c dx dy dz in that order; part[2 b + 0/1] = workgroup sums (fixed order).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_mass(const float *__restrict__ chi, const float *__restrict__ clo,
+                                              const double *__restrict__ c64, int nx, int ny, int nz, int nb,
+                                              MassSpacings sp, double *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int nrows = ny * nz * nb;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  double mv = 0.0, mc = 0.0;
+  for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < nrows; r += nw) {
+    const RowWalk w = row_of(r, nx, ny, nz);
+    const double pj = sp.p2[w.j + 1], pk = sp.p3[w.k + 1];
+    const double bj = sp.b2[w.j], bk = sp.b3[w.k];
+#pragma unroll 4
+    for (int i = lane + 1; i <= nx; i += 64) {
+      const double v = state_at<MODE>(chi, clo, c64, w.base + (size_t)i);
+      mv += v * sp.p1[i + 1] * pj * pk;
+      mc += v * sp.b1[i] * bj * bk;
+    }
+  }
+  __shared__ double red[2][4];
+  const int wid = threadIdx.x >> 6;
+  mv = wsum(mv);
+  mc = wsum(mc);
+  if (lane == 0) {
+    red[0][wid] = mv;
+    red[1][wid] = mc;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double r = 0.0;
-    for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) r += red[w];
-    out[0] = r;
+  if (threadIdx.x < 2) {
+    double s = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[threadIdx.x][q];
+    part[2 * (size_t)blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+// out[q] = sum over b < nblk of part[2 b + q], q = 0, 1 (fixed order)
+__global__ void k_mass_final(const double *__restrict__ part, int nblk, double *__restrict__ out) {
+  __shared__ double red[2][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[2 * (size_t)b + q];
+    s = wsum(s);
+    if (lane == 0) red[q][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[threadIdx.x][w];
+    out[threadIdx.x] = s;
   }
 }
 
 hipError_t mass(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,
-                const double *d1, const double *d2, const double *d3, int off, double *part, int part_cap,
-                double *out, hipStream_t st) {
+                const MassSpacings &sp, double *part, int part_cap, double *out, hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
-  const size_t n = (size_t)nx * ny * nz * nb;
-  int g = (int)((n + 255) / 256);
-  if (g > part_cap) g = part_cap;
+  const size_t rows = (size_t)ny * nz * nb;  // one wave per row, 4 waves per workgroup
+  if (rows > (size_t)INT_MAX) return hipErrorInvalidValue;
+  int g = (int)((rows + 3) / 4);
+  if (g > part_cap / 2) g = part_cap / 2;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL(k_mass, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, d1, d2, d3, off, part);
-  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, part, g, out);
+  if (c64)
+    hipLaunchKernelGGL(k_mass<2>, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, sp, part);
+  else if (clo)
+    hipLaunchKernelGGL(k_mass<1>, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, sp, part);
+  else
+    hipLaunchKernelGGL(k_mass<0>, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, sp, part);
+  hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(256), 0, st, part, g, out);
   return hipGetLastError();
 }
 
@@ -712,35 +766,28 @@ hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st) {
 // ---------------------------------------------------------------------------
 namespace f3d {
 
+template <int MODE>
 __global__ void __launch_bounds__(256) k_health(const float *__restrict__ chi, const float *__restrict__ clo,
                                                 const double *__restrict__ c64, int nx, int ny, int nz, int nb,
                                                 double *__restrict__ part) {
-  const size_t ncell = (size_t)nx * ny * nz * nb;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nrows = ny * nz * nb;
+  const int nw = gridDim.x * (blockDim.x >> 6);
   double nonfin = 0.0, neg = 0.0, mn = INFINITY;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx) + 1;
-    size_t r = c / nx;
-    const int j = (int)(r % ny) + 1;
-    r /= ny;
-    const int k = (int)(r % nz) + 1;
-    const int b = (int)(r / nz);
-    const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
-    double v;
-    if (c64) {
-      v = c64[ci];
-    } else {
-      v = (double)chi[ci];
-      if (clo) v += (double)clo[ci];
-    }
-    if (!isfinite(v)) {
-      nonfin += 1.0;
-    } else {
-      if (v < 0.0) neg += 1.0;
-      mn = fmin(mn, v);
+  for (int r = blockIdx.x * (blockDim.x >> 6) + wid; r < nrows; r += nw) {
+    const RowWalk w = row_of(r, nx, ny, nz);
+#pragma unroll 4
+    for (int i = lane + 1; i <= nx; i += 64) {
+      const double v = state_at<MODE>(chi, clo, c64, w.base + (size_t)i);
+      if (!isfinite(v)) {
+        nonfin += 1.0;
+      } else {
+        if (v < 0.0) neg += 1.0;
+        mn = fmin(mn, v);
+      }
     }
   }
   __shared__ double red[3][4];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int o = 32; o > 0; o >>= 1) {
     nonfin += __shfl_down(nonfin, o, 64);
     neg += __shfl_down(neg, o, 64);
@@ -782,11 +829,17 @@ __global__ void k_health_final(const double *__restrict__ part, int nblk, double
 hipError_t health(const float *chi, const float *clo, const double *c64, int nx, int ny, int nz, int nb,
                   double *part, int part_cap, double *out, hipStream_t st) {
   (void)hipGetLastError();
-  const size_t n = (size_t)nx * ny * nz * nb;
-  int g = (int)((n + 255) / 256);
+  const size_t rows = (size_t)ny * nz * nb;  // one wave per row, 4 waves per workgroup
+  if (rows > (size_t)INT_MAX) return hipErrorInvalidValue;
+  int g = (int)((rows + 3) / 4);
   if (g > part_cap / 3) g = part_cap / 3;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL(k_health, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, part);
+  if (c64)
+    hipLaunchKernelGGL(k_health<2>, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, part);
+  else if (clo)
+    hipLaunchKernelGGL(k_health<1>, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, part);
+  else
+    hipLaunchKernelGGL(k_health<0>, dim3(g), dim3(256), 0, st, chi, clo, c64, nx, ny, nz, nb, part);
   hipLaunchKernelGGL(k_health_final, dim3(1), dim3(64), 0, st, part, g, out);
   return hipGetLastError();
 }

@@ -310,8 +310,22 @@ int run_sweep(fall3d_ads_ctx *c, int dir, double dt, double stime, const float *
   return FALL3D_ADS_OK;
 }
 
+// dst = src with its two fastest axes swapped (A x B planes, K of them),
+// in launches of at most 65535 planes
+hipError_t transpose64(const double *src, double *dst, int A, int B, size_t K, hipStream_t st) {
+  const size_t plane = (size_t)A * B;
+  for (size_t k0 = 0; k0 < K; k0 += 65535) {
+    const int kk = (int)(K - k0 < 65535 ? K - k0 : 65535);
+    hipError_t e = transpose_to_f64(src + k0 * plane, 8, dst + k0 * plane, A, B, kk, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 // FP64 state: one sweep with the given FP64 winds (vel1, vel2: lerped by
-// stime), diffusion and, for z, settling velocity
+// stime), diffusion and, for z, settling velocity. The x sweep runs on
+// (x, y)-transposed copies of the state, winds and diffusion, and the state
+// is transposed back after it.
 int run_sweep64(fall3d_ads_ctx *c, int dir, double dt, double stime, const double *vel1, const double *vel2,
                 const double *kd, SweepOut *out) {
   Sweep64Args a;
@@ -340,8 +354,25 @@ int run_sweep64(fall3d_ads_ctx *c, int dir, double dt, double stime, const doubl
   const int nblocks = sweep64_blocks(a.nlines, c->nb);
   {
     ProfScope ps(c, KID_SWEEP_X + dir);
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const size_t planes = (size_t)(nz + 4) * c->nb;
+    if (dir == 0) {
+      hipError_t e = transpose64(c->c64, c->c64t, nx + 4, ny + 4, planes, c->stream);
+      if (e == hipSuccess) e = transpose64(vel1, c->u1t, nx + 3, ny, nz, c->stream);
+      if (e == hipSuccess && vel2 != vel1) e = transpose64(vel2, c->u2t, nx + 3, ny, nz, c->stream);
+      if (e == hipSuccess) e = transpose64(kd, c->k1t, nx + 4, ny, nz, c->stream);
+      if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "FP64 x-sweep transposes: %s", hipGetErrorString(e));
+      a.c = c->c64t;
+      a.vel1 = c->u1t;
+      a.vel2 = vel2 != vel1 ? c->u2t : c->u1t;
+      a.kd = c->k1t;
+    }
     hipError_t e = launch_sweep64(dir, c->limiter, a, nblocks, c->stream);
     if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "FP64 sweep %d launch: %s", dir, hipGetErrorString(e));
+    if (dir == 0) {
+      e = transpose64(c->c64t, c->c64, ny + 4, nx + 4, planes, c->stream);
+      if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "FP64 x-sweep transpose: %s", hipGetErrorString(e));
+    }
   }
   out->nblocks = nblocks;
   return FALL3D_ADS_OK;
@@ -701,6 +732,10 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     double **m64[10] = {&c->u1d, &c->u2d, &c->v1d, &c->v2d, &c->w1d, &c->w2d, &c->k1d, &c->k2d, &c->k3d, &c->vsd};
     const size_t n64[10] = {n_u(c), n_u(c), n_v(c), n_v(c), n_w(c), n_w(c), n_k1(c), n_k2(c), n_k3(c), n_vs(c)};
     for (int q = 0; q < 10; ++q) HIPCHK(c, dalloc(m64[q], n64[q]));
+    HIPCHK(c, dalloc(&c->c64t, c->ncell_c));
+    HIPCHK(c, dalloc(&c->u1t, n_u(c)));
+    HIPCHK(c, dalloc(&c->u2t, n_u(c)));
+    HIPCHK(c, dalloc(&c->k1t, n_k1(c)));
   } else {
     HIPCHK(c, dalloc(&c->chi_base, c->ncell_c + 2));
     c->chi = c->chi_base + 2;
@@ -762,7 +797,7 @@ int fall3d_ads_release(fall3d_ads_ctx *c) {
                   c->src_idx, c->src_rate, c->src_meas, c->src_part, c->acc, c->part[1], c->part[2],
                   c->part[3], c->rpart[1], c->rpart[2], c->rpart[3], c->red_part,
                   c->dstage, c->c64, c->u1d, c->u2d, c->v1d, c->v2d, c->w1d, c->w2d, c->k1d, c->k2d, c->k3d,
-                  c->vsd, c->src_s, c->src_vol};  // part[0] aliases src_part
+                  c->vsd, c->c64t, c->u1t, c->u2t, c->k1t, c->src_s, c->src_vol};  // part[0] aliases src_part
   for (void *p : ptrs) dfree(p);
   for (int d = 0; d < 3; ++d) {
     dfree(c->dxp64[d]);
@@ -1225,20 +1260,17 @@ int fall3d_ads_get_budget(fall3d_ads_ctx *c, double out[FALL3D_ADS_BUDGET_LEN]) 
   int rc;
   if ((rc = need_grid(c))) return rc;
   double *o = c->red_part + (c->red_cap - 2);
-  // mass with mass-point spacings (dxp position p at index p+1) and with cell sizes
-  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->c64, c->nx, c->ny, c->nz, c->nb, c->dxp64[0], c->dxp64[1],
-                 c->dxp64[2], 1, c->red_part, c->red_cap - 2, o, c->stream));
-  double mv = 0.0, mc = 0.0;
-  HIPCHK(c, hipMemcpyAsync(&mv, o, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->c64, c->nx, c->ny, c->nz, c->nb, c->dxb64[0], c->dxb64[1],
-                 c->dxb64[2], 0, c->red_part, c->red_cap - 2, o, c->stream));
-  HIPCHK(c, hipMemcpyAsync(&mc, o, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  // mass with mass-point spacings and with cell sizes, in one pass
+  const MassSpacings sp = {c->dxp64[0], c->dxp64[1], c->dxp64[2], c->dxb64[0], c->dxb64[1], c->dxb64[2]};
+  HIPCHK(c, mass(c->chi, c->comp ? c->clo : nullptr, c->c64, c->nx, c->ny, c->nz, c->nb, sp, c->red_part,
+                 c->red_cap - 2, o, c->stream));
+  double m2[2] = {0.0, 0.0};
+  HIPCHK(c, hipMemcpyAsync(m2, o, sizeof(m2), hipMemcpyDeviceToHost, c->stream));
   double acc[ACC_LEN];
   HIPCHK(c, hipMemcpyAsync(acc, c->acc, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  out[FALL3D_ADS_BUDGET_MASS_VOLUME] = mv;
-  out[FALL3D_ADS_BUDGET_MASS_CELLS] = mc;
+  out[FALL3D_ADS_BUDGET_MASS_VOLUME] = m2[0];
+  out[FALL3D_ADS_BUDGET_MASS_CELLS] = m2[1];
   out[FALL3D_ADS_BUDGET_GROUND] = fabs(acc[ACC_FLUX + 4]);
   out[FALL3D_ADS_BUDGET_LATERAL] =
       fabs(acc[ACC_FLUX + 0]) + fabs(acc[ACC_FLUX + 1]) + fabs(acc[ACC_FLUX + 2]) + fabs(acc[ACC_FLUX + 3]) +

@@ -64,6 +64,11 @@ struct fall3d_ads_ctx {
   double *c64 = nullptr;
   double *u1d = nullptr, *u2d = nullptr, *v1d = nullptr, *v2d = nullptr, *w1d = nullptr, *w2d = nullptr;
   double *k1d = nullptr, *k2d = nullptr, *k3d = nullptr, *vsd = nullptr;
+  // the FP64 x sweep runs on (x, y)-transposed copies of the state and of its
+  // fields (y fastest), so that the lanes of a wave (consecutive y lines)
+  // read adjacent words: c64t (-1:ny+2,-1:nx+2,-1:nz+2,nb), u1t/u2t
+  // (ny,0:nx+2,nz), k1t (ny,-1:nx+2,nz)
+  double *c64t = nullptr, *u1t = nullptr, *u2t = nullptr, *k1t = nullptr;
   double *acum = nullptr;
   size_t ncell_c = 0;  // (nx+4)(ny+4)(nz+4)nb
   size_t nacum = 0;    // (nx+4)(ny+4)nb

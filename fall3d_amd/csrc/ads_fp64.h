@@ -9,9 +9,12 @@ namespace f3d {
 // cells and 4 ghosts) of one bin; lines are numbered with the fastest
 // transverse index first (x sweep: j then k; y sweep: i then k; z sweep:
 // i then j), bins outermost. Device layouts of vel/kd/vs are those of the
-// FP32 fields (ads_internal.h), in FP64.
+// FP32 fields (ads_internal.h), in FP64, except for the x sweep, which
+// reads the state, wind and diffusion with their two fastest axes swapped
+// (c(-1:ny+2,-1:nx+2,-1:nz+2,nb), vel(ny,0:nx+2,nz), kd(ny,-1:nx+2,nz)), so
+// that neighbouring lines (consecutive j) are adjacent in memory.
 struct Sweep64Args {
-  double *c;                  // c(-1:nx+2,-1:ny+2,-1:nz+2,nb)
+  double *c;                  // c(-1:nx+2,-1:ny+2,-1:nz+2,nb) (x sweep: transposed, see above)
   int nx, ny, nz, nb;
   int n;                      // cells along the sweep axis
   int tm;                     // 1 Euler, 2 RK4

@@ -115,13 +115,14 @@ __global__ void __launch_bounds__(256) k_sweep64(Sweep64Args a) {
     ptrdiff_t cb, cs;
     size_t vb, vst, kb, sb = 0;
     if constexpr (DIR == 0) {
+      // (x, y)-transposed layouts: c(j, p, k, b), wind(j, f, k), kd(j, p, k)
       t1 = (int)(L % ny) + 1;
       t2 = (int)(L / ny) + 1;
-      cb = (ptrdiff_t)(1 + (size_t)(nx + 4) * ((size_t)(t1 + 1) + (size_t)(ny + 4) * ((size_t)(t2 + 1) + (size_t)(nz + 4) * b)));
-      cs = 1;
-      vb = (size_t)(nx + 3) * L;
-      kb = (size_t)(nx + 4) * L;
-      vst = 1;
+      cb = (ptrdiff_t)((size_t)(t1 + 1) + (size_t)(ny + 4) * (1 + (size_t)(nx + 4) * ((size_t)(t2 + 1) + (size_t)(nz + 4) * b)));
+      cs = ny + 4;
+      vb = (size_t)(t1 - 1) + (size_t)ny * (nx + 3) * (t2 - 1);
+      kb = (size_t)(t1 - 1) + (size_t)ny * (nx + 4) * (t2 - 1);
+      vst = ny;
     } else if constexpr (DIR == 1) {
       t1 = (int)(L % nx) + 1;
       t2 = (int)(L / nx) + 1;
