@@ -225,21 +225,6 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-// Block sum of one double per thread into out[blockIdx.x] (fixed order).
-__device__ void block_sum_store(double v, double *out) {
-  __shared__ double red[16];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  v = wsum(v);
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < nw; ++w) s += red[w];
-    out[blockIdx.x] = s;
-  }
-}
-
 // Each entry: c[idx] += dt * rate_vol (rate_vol = s/vol in FP64); the
 // injected mass (increment times the cell's mass measure) is summed into
 // partials[blockIdx.x*4 + 0] (slots 1..3 zero, same layout as the sweeps).

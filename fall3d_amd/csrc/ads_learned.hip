@@ -37,6 +37,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "ads_context.h"
 #include "ads_mlp.h"
 #include "ads_mlp_device.h"
@@ -607,6 +609,35 @@ __global__ void __launch_bounds__(256) k_collect_features(FaceArgs a, uint16_t *
   }
 }
 
+// Canonical order of the rows of one collection: a face's key orders by bin,
+// axis, k, transverse index and face index (radices nk, nt, nf above their
+// largest values), so the rows, and the batches sampled from them, do not
+// depend on the order in which waves appended them.
+__global__ void k_row_keys(const int4 *__restrict__ ref, long m, int nk, int nt, int nf,
+                           unsigned long long *__restrict__ key, unsigned int *__restrict__ idx) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int4 q = ref[i];
+  const unsigned long long ba = 2ull * (unsigned long long)(q.x >> 1) + (unsigned long long)(q.x & 1);
+  key[i] = ((ba * (unsigned)nk + (unsigned)q.w) * (unsigned)nt + (unsigned)q.z) * (unsigned)nf + (unsigned)q.y;
+  idx[i] = (unsigned int)i;
+}
+
+// row i of the sorted order := row perm[i] (features, face, denominator)
+__global__ void k_gather_rows(const uint16_t *__restrict__ xs, const int4 *__restrict__ rs,
+                              const float *__restrict__ ds, const unsigned int *__restrict__ perm, long m,
+                              uint16_t *__restrict__ xd, int4 *__restrict__ rd, float *__restrict__ dd) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const long src = perm[i];
+  const uint4 *xsrc = reinterpret_cast<const uint4 *>(xs + src * 32);
+  uint4 *xdst = reinterpret_cast<uint4 *>(xd + i * 32);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) xdst[q] = xsrc[q];
+  rd[i] = rs[src];
+  dd[i] = ds[src];
+}
+
 // targets of rows r0..r1: (fine flux averaged over the two fine faces of
 // the coarse face - coarse flux) / (kOutScale den)
 __global__ void k_collect_targets(const int4 *__restrict__ ref, const float *__restrict__ den, long r0, long r1,
@@ -858,6 +889,58 @@ void learned_release(fall3d_ads_ctx *c) {
 
 }  // namespace host
 }  // namespace f3d
+
+// Rows r0..r1 of n into their canonical order (k_row_keys): a radix sort of
+// the face keys, then a gather through temporary buffers.
+static int sort_rows(fall3d_mlp *n, fall3d_ads_ctx *coarse, long r0, long r1, hipStream_t st) {
+  const long m = r1 - r0;
+  if (m < 2) return FALL3D_ADS_OK;
+  if (m > (long)INT_MAX) return nfail(n, FALL3D_ADS_EINVAL, "too many rows in one collection (%ld)", m);
+  const int nk = coarse->nz + 1, nt = std::max(coarse->nx, coarse->ny) + 1, nf = nt;
+  const double kmax = 2.0 * coarse->nb * (double)nk * nt * nf;
+  if (kmax >= 18446744073709551615.0) return nfail(n, FALL3D_ADS_EINVAL, "face keys exceed 64 bits");
+  int nbits = 1;
+  while (nbits < 64 && std::ldexp(1.0, nbits) < kmax) ++nbits;
+  unsigned long long *key = nullptr, *key2 = nullptr;
+  unsigned int *idx = nullptr, *idx2 = nullptr;
+  uint16_t *xt = nullptr;
+  int4 *rt = nullptr;
+  float *dtmp = nullptr;
+  void *tmp = nullptr;
+  size_t tbytes = 0;
+  hipError_t e = hipSuccess;
+  auto take = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  take(hipMalloc(&key, m * sizeof(*key)));
+  take(hipMalloc(&key2, m * sizeof(*key2)));
+  take(hipMalloc(&idx, m * sizeof(*idx)));
+  take(hipMalloc(&idx2, m * sizeof(*idx2)));
+  take(hipMalloc(&xt, m * 32 * sizeof(*xt)));
+  take(hipMalloc(&rt, m * sizeof(*rt)));
+  take(hipMalloc(&dtmp, m * sizeof(*dtmp)));
+  if (e == hipSuccess) {
+    const unsigned g = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(k_row_keys, dim3(g), dim3(256), 0, st, n->ref + r0, m, nk, nt, nf, key, idx);
+    take(hipGetLastError());
+    take(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, key, key2, idx, idx2, (int)m, 0, nbits, st));
+    take(hipMalloc(&tmp, tbytes > 0 ? tbytes : 1));
+    take(hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, key, key2, idx, idx2, (int)m, 0, nbits, st));
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_gather_rows, dim3(g), dim3(256), 0, st, n->x + r0 * 32, n->ref + r0, n->den + r0, idx2, m,
+                         xt, rt, dtmp);
+      take(hipGetLastError());
+      take(hipMemcpyAsync(n->x + r0 * 32, xt, m * 32 * sizeof(*xt), hipMemcpyDeviceToDevice, st));
+      take(hipMemcpyAsync(n->ref + r0, rt, m * sizeof(*rt), hipMemcpyDeviceToDevice, st));
+      take(hipMemcpyAsync(n->den + r0, dtmp, m * sizeof(*dtmp), hipMemcpyDeviceToDevice, st));
+      take(hipStreamSynchronize(st));
+    }
+  }
+  for (void *p : {(void *)key, (void *)key2, (void *)idx, (void *)idx2, (void *)xt, (void *)rt, (void *)dtmp, tmp})
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) return nfail(n, FALL3D_ADS_EHIP, "row sort: %s", hipGetErrorString(e));
+  return FALL3D_ADS_OK;
+}
 
 extern "C" {
 
@@ -1154,6 +1237,7 @@ int fall3d_ads_learned_collect(fall3d_ads_ctx *fine, fall3d_ads_ctx *coarse, fal
   HIPCHK(coarse, hipMemcpyAsync(&cnt, n->dcount, sizeof(cnt), hipMemcpyDeviceToHost, st));
   HIPCHK(coarse, hipStreamSynchronize(st));
   const long r1 = std::min<long>((long)cnt, n->cap);
+  if ((rc = sort_rows(n, coarse, r0, r1, st))) return rc;
   // 3. one plain coarse step and two fine steps with face-flux output
   fall3d_mlp *keep_c = coarse->learned, *keep_f = fine->learned;
   coarse->learned = fine->learned = nullptr;

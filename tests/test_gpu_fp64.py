@@ -59,6 +59,27 @@ def test_dropin_sweeps_bitwise(lim, tm, grid):
     g.release()
 
 
+def test_dropin_x_sweep_more_planes_than_one_transpose_launch():
+    """The FP64 x sweep runs on an (x, y)-transposed copy of the state; a
+    transpose launch covers at most 65535 (z, bin) planes, so 6 z positions
+    (nz + 4) x 10923 bins = 65538 planes take two launches each way."""
+    print("case fp64 x sweep 6x4x2x10923", flush=True)
+    p = random_problem(6, 4, 2, 10923, seed=5)
+    assert (p.nz + 4) * p.nb > 65535
+    o = Oracle(p, 1, 2)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, 1, 2, 8, "fp64", grid=p.grid)
+    u, _, _ = g.get_vcomps()
+    u[...] = p.u1
+    dt = o.time_step(p.u1, p.v1, p.w1, p.vs, criterion=2, cfl=0.5)
+    cr, cg = p.new_c(), p.new_c()
+    o.solve_x(dt, cr, p.k1, p.u1)
+    g.solve_along_x(0.0, 0.0, dt, cg, p.k1)
+    _equal(cg, cr, "x sweep c, 65538 planes")
+    b = g.budget()
+    assert b["mass_volume"] == pytest.approx(o.mass_volume(cr), rel=1e-13)
+    g.release()
+
+
 def test_dropin_real4_host_arrays():
     """rp = 4 host arrays into the FP64 state: the inputs widen exactly, so
     the sweep is the FP64 sweep of the widened field."""

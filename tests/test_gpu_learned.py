@@ -29,6 +29,30 @@ def test_collected_rows_are_finite_and_consistent():
     net.release()
 
 
+def test_collection_and_training_are_reproducible():
+    """Rows are kept in a canonical face order (not the order in which waves
+    appended them), so two collections of the same problem store the same
+    rows in the same order, and training from the same seed gives the same
+    loss curve bit for bit."""
+    nets = [ads.Network(capacity=400_000, seed=4) for _ in range(2)]
+    counts = [learned.collect(n, [1], SIZE, nwarm=10, nsamples=2, gap=1, **KW) for n in nets]
+    print(f"rows {counts}", flush=True)
+    assert counts[0] == counts[1] > 1000
+    (x0, t0), (x1, t1) = nets[0].rows(), nets[1].rows()
+    (r0, d0), (r1, d1) = nets[0].refs(), nets[1].refs()
+    assert np.array_equal(x0, x1) and np.array_equal(t0, t1)
+    assert np.array_equal(r0, r1) and np.array_equal(d0, d1)
+    # each of the 2 collection calls stores its faces in strictly increasing
+    # key order (bin, axis, k, cross, face): at most one descent, no repeats
+    key = [tuple(q) for q in np.column_stack([r0[:, 0] >> 1, r0[:, 0] & 1, r0[:, 3], r0[:, 2], r0[:, 1]])]
+    assert sum(key[i + 1] < key[i] for i in range(len(key) - 1)) <= 1
+    assert all(key[i + 1] != key[i] for i in range(len(key) - 1))
+    losses = [n.train(50, batch=8192, lr=1e-3, seed=7) for n in nets]
+    assert np.array_equal(losses[0], losses[1])
+    for n in nets:
+        n.release()
+
+
 def test_rows_match_the_cpu_restatement():
     """Every collected row: features bit-identical to oracle/learned_oracle.py
     on the coarse-grained state, targets equal to the restatement from the
