@@ -13,7 +13,7 @@
 //   per[i]    r(min(max(int(p_i/100 nmem), 1), nmem)) (:415-437)
 //   prob[i]   100 #{m : x_m >= th_i} / nmem          (:388-410)
 // The reference's standard-deviation loop assigns instead of accumulating
-// (:357-361), so it would keep only the last member's squared deviation;
+// (:356-360), so it would keep only the last member's squared deviation;
 // this computes the sum its division by nens-1 is written for.
 //
 // Up to 64 members a thread owns a point and sorts its values with a

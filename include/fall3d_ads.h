@@ -441,7 +441,7 @@ int fall3d_ads_ensemble_autotune(fall3d_ads_ensemble *ens, const double *dt, con
  *   mean[p]                 sum_m x / nmem
  *   logmean[p]              10^(sum_m log10 x / nmem)
  *   stddev[p]               sqrt(sum_m (x - mean)^2 / (nmem - 1)) (the sum
- *                           the reference's loop at :357-361 is written for;
+ *                           the reference's loop at :356-360 is written for;
  *                           that loop keeps only the last member's term)
  *   median[p]               r(int(nmem/2)), r the member values in ascending
  *                           order (postp_rank_vector, Sources/mod_Postp.F90:185), 1-based
