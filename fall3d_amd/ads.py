@@ -51,7 +51,7 @@ EXPORTS = (
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
     "fall3d_ads_copy_probe", "fall3d_ads_check_state", "fall3d_ads_learned_fallbacks", "fall3d_ads_snapshot",
     "fall3d_ads_restore", "fall3d_ens_statistics", "fall3d_ads_update_meteo", "fall3d_ads_get_meteo",
-    "fall3d_ads_postprocess", "fall3d_ads_set_boundary",
+    "fall3d_ads_postprocess", "fall3d_ads_postprocess_cuts", "fall3d_ads_set_boundary",
     "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
     "fall3d_ads_ensemble_create", "fall3d_ads_ensemble_release", "fall3d_ads_ensemble_last_error",
     "fall3d_ads_ensemble_member", "fall3d_ads_ensemble_step", "fall3d_ads_ensemble_run",
@@ -94,6 +94,14 @@ class _OutputArgs(ctypes.Structure):
                 ("z_c", ctypes.c_void_p), ("bin_species", ctypes.c_void_p), ("species", ctypes.c_int),
                 ("species_category", ctypes.c_int), ("so2_du", ctypes.c_int), ("npm", ctypes.c_int),
                 ("pm_um", ctypes.c_void_p), ("bin_diam", ctypes.c_void_p)]
+
+
+class _CutArgs(ctypes.Structure):
+    _fields_ = [("lonmin", ctypes.c_double), ("dlon", ctypes.c_double), ("latmin", ctypes.c_double),
+                ("dlat", ctypes.c_double), ("sigma", ctypes.c_void_p), ("X3max", ctypes.c_double),
+                ("ncutx", ctypes.c_int), ("ncuty", ctypes.c_int), ("ncutz", ctypes.c_int), ("nfl", ctypes.c_int),
+                ("x_cut", ctypes.c_void_p), ("y_cut", ctypes.c_void_p), ("z_cut", ctypes.c_void_p),
+                ("fl_cut", ctypes.c_void_p)]
 
 
 METEO_MAP_H = {"cartesian": 0, "spherical": 1}
@@ -157,6 +165,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_update_meteo": ([_vp, _vp, _vp, _vp], _int),
         "fall3d_ads_get_meteo": ([_vp] * 11, _int),
         "fall3d_ads_postprocess": ([_vp] * 10, _int),
+        "fall3d_ads_postprocess_cuts": ([_vp] * 7, _int),
         "fall3d_ads_set_boundary": ([_vp, _int], _int),
         "fall3d_ads_run": ([_vp, _dbl, _dbl, _int, _int], _int),
         "fall3d_ads_get_fluxes": ([_vp, _dp], _int),
@@ -470,6 +479,23 @@ class ADS:
         """nc_IO_out_res's result variables from the device state (see
         fall3d_ads_postprocess); returns a dict of FP64 corner arrays."""
         nx, ny, nz, nb = self.nx, self.ny, self.nz, self.nb
+        a, keep = self._output_args(z_c, hm1, hm2, hm3, bin_species, species, species_category, so2_du, pm_um,
+                                    bin_diam)
+        pm = keep[-1]
+        c2, c3 = (nx + 1, ny + 1), (nx + 1, ny + 1, nz + 1)
+        shapes = {"con_bins": c3 + (nb,), "con": c3, "con_sur": c2, "col_mass": c2,
+                  "col_mass_pm": c2 + (max(pm.size, 1),), "cloud_top": c2, "grn_load": c2, "grn_load_bin": c2 + (nb,)}
+        out = {k: np.zeros(shapes[k], order="F") for k in want}
+        ptr = lambda k: out[k].ctypes.data if k in out else None
+        self._chk(load_library().fall3d_ads_postprocess(self._h, ctypes.byref(a), *(ptr(k) for k in shapes)),
+                  "postprocess")
+        del keep
+        return out
+
+    def _output_args(self, z_c, hm1, hm2, hm3, bin_species, species, species_category, so2_du, pm_um, bin_diam):
+        """fall3d_output_args and the arrays it points to (kept alive by the
+        caller; the last one is the PM threshold array)."""
+        nx, ny, nz = self.nx, self.ny, self.nz
         keep = []
 
         def arr(x, shape, name):
@@ -486,14 +512,30 @@ class ADS:
                         arr(z_c, (nx + 1, ny + 1, nz + 1), "z_c"), None if spe is None else spe.ctypes.data,
                         int(species), int(species_category), int(bool(so2_du)), int(pm.size),
                         pm.ctypes.data if pm.size else None, None if diam is None else diam.ctypes.data)
-        c2, c3 = (nx + 1, ny + 1), (nx + 1, ny + 1, nz + 1)
-        shapes = {"con_bins": c3 + (nb,), "con": c3, "con_sur": c2, "col_mass": c2,
-                  "col_mass_pm": c2 + (max(pm.size, 1),), "cloud_top": c2, "grn_load": c2, "grn_load_bin": c2 + (nb,)}
-        out = {k: np.zeros(shapes[k], order="F") for k in want}
-        ptr = lambda k: out[k].ctypes.data if k in out else None
-        self._chk(load_library().fall3d_ads_postprocess(self._h, ctypes.byref(a), *(ptr(k) for k in shapes)),
-                  "postprocess")
-        del keep
+        keep += [spe, diam, pm]
+        return a, keep
+
+    def postprocess_cuts(self, z_c, sigma, X3max, lonmin=0.0, dlon=1.0, latmin=0.0, dlat=1.0, x_cut=(), y_cut=(),
+                         z_cut=(), fl_cut=(), hm1=None, hm2=None, hm3=None, bin_species=None, species=0):
+        """nc_IO_out_res's concentration cuts of one species (see
+        fall3d_ads_postprocess_cuts): dict with cut_x (ny+1, nz+1, ncutx),
+        cut_y (nx+1, nz+1, ncuty), cut_z (nx+1, ny+1, ncutz) and cut_fl
+        (nx+1, ny+1, nfl) in g/m3; fl_cut in flight levels."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        a, keep = self._output_args(z_c, hm1, hm2, hm3, bin_species, species, 1, False, (), None)
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        if sg.shape != (nz + 1,):
+            raise ValueError(f"sigma: expected {nz + 1} corner levels, got shape {sg.shape}")
+        pos = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (x_cut, y_cut, z_cut, fl_cut)]
+        ca = _CutArgs(float(lonmin), float(dlon), float(latmin), float(dlat), sg.ctypes.data, float(X3max),
+                      *(int(v.size) for v in pos), *(v.ctypes.data if v.size else None for v in pos))
+        shapes = [(ny + 1, nz + 1), (nx + 1, nz + 1), (nx + 1, ny + 1), (nx + 1, ny + 1)]
+        out = {k: np.zeros(sh + (v.size,), order="F")
+               for k, sh, v in zip(("cut_x", "cut_y", "cut_z", "cut_fl"), shapes, pos)}
+        ptrs = [o.ctypes.data if o.size else None for o in out.values()]
+        self._chk(load_library().fall3d_ads_postprocess_cuts(self._h, ctypes.byref(a), ctypes.byref(ca), *ptrs),
+                  "postprocess_cuts")
+        del keep, sg, pos
         return out
 
     def set_boundary(self, mode="freeflow"):

@@ -132,17 +132,108 @@ __global__ void __launch_bounds__(256) k_output(OutArgs a) {
   }
 }
 
+// grid_get_shapez (Sources/mod_Grid.F90:1287-1329) over v(1..n) (element
+// q at v[(q-1) * stride]): the interval [v(k), v(k+1)] holding z, searched
+// from the bottom, and the linear weight of v(k+1); below v(1) it gives
+// (1, 0), above v(n) (n-1, 1).
+__host__ __device__ inline void shapez(const double *v, size_t stride, int n, double z, int &ik, double &s) {
+  if (z < v[0]) {
+    ik = 1;
+    s = 0.0;
+    return;
+  }
+  if (z > v[(size_t)(n - 1) * stride]) {
+    ik = n - 1;
+    s = 1.0;
+    return;
+  }
+  for (int k = 1; k < n; ++k) {
+    const double lo = v[(size_t)(k - 1) * stride], hi = v[(size_t)k * stride];
+    if (z >= lo && z <= hi) {
+      ik = k;
+      s = (z - lo) / (hi - lo);
+      return;
+    }
+  }
+  ik = 1;  // only for a column that is not ordered upward (the reference never ends its search then)
+  s = 0.0;
+}
+
+// the species' corner concentration on the z levels zl(1..nz+1) above sea
+// level: per corner column, interpolated over the column's own corner
+// heights (mod_nc_IO.F90:3302-3313 and 4102-4117)
+__global__ void __launch_bounds__(256) k_zlevels(const double *__restrict__ con, const double *__restrict__ zc,
+                                                 const double *__restrict__ zl, int nx, int ny, int nz,
+                                                 double *__restrict__ cz) {
+  const size_t ncol = (size_t)(nx + 1) * (ny + 1);
+  for (size_t col = blockIdx.x * (size_t)blockDim.x + threadIdx.x; col < ncol; col += (size_t)gridDim.x * blockDim.x) {
+    for (int k = 1; k <= nz + 1; ++k) {
+      int kk;
+      double sh;
+      shapez(zc + col, ncol, nz + 1, zl[k - 1], kk, sh);
+      cz[col + ncol * (size_t)(k - 1)] =
+          kk > 0 ? (1.0 - sh) * con[col + ncol * (size_t)(kk - 1)] + sh * con[col + ncol * (size_t)kk] : 0.0;
+    }
+  }
+}
+
+// Cuts of cz (mod_nc_IO.F90:4126-4332): mode 0 x cuts (ny+1, nz+1, ncut),
+// 1 y cuts (nx+1, nz+1, ncut), 2 z or flight-level cuts (nx+1, ny+1, ncut);
+// cut q interpolates between positions idx[q] and idx[q]+1 with weight sh[q].
+__global__ void __launch_bounds__(256) k_cuts(const double *__restrict__ cz, int nx, int ny, int nz, int mode,
+                                              int ncut, const int *__restrict__ idx, const double *__restrict__ sh,
+                                              double *__restrict__ out) {
+  const size_t n1 = (size_t)(mode == 0 ? ny + 1 : nx + 1), n2 = (size_t)(mode == 2 ? ny + 1 : nz + 1);
+  const size_t total = n1 * n2 * (size_t)ncut;
+  const size_t sx = 1, sy = (size_t)(nx + 1), sz = (size_t)(nx + 1) * (ny + 1);
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t a = e % n1, r = e / n1, b = r % n2;
+    const int q = (int)(r / n2);
+    const size_t m = (size_t)(idx[q] - 1);
+    size_t lo, step;
+    if (mode == 0) {  // (j, k) at x position m
+      lo = m * sx + a * sy + b * sz;
+      step = sx;
+    } else if (mode == 1) {  // (i, k) at y position m
+      lo = a * sx + m * sy + b * sz;
+      step = sy;
+    } else {  // (i, j) at level m
+      lo = a * sx + b * sy + m * sz;
+      step = sz;
+    }
+    out[e] = (1.0 - sh[q]) * cz[lo] + sh[q] * cz[lo + step];
+  }
+}
+
 }  // namespace
 }  // namespace f3d
 
 using namespace f3d;
 using namespace f3d::host;
 
-extern "C" int fall3d_ads_postprocess(fall3d_ads_ctx *c, const fall3d_output_args *o, double *con_bins, double *con,
-                                      double *con_sur, double *col_mass, double *col_mass_pm, double *cloud_top,
-                                      double *grn_load, double *grn_load_bin) {
+namespace {
+
+// cut request of fall3d_ads_postprocess_cuts: host outputs x, y, z, FL
+struct CutReq {
+  const fall3d_cut_args *args;
+  double *out[4];
+};
+
+int postprocess_impl(fall3d_ads_ctx *c, const fall3d_output_args *o, double *con_bins, double *con, double *con_sur,
+                     double *col_mass, double *col_mass_pm, double *cloud_top, double *grn_load,
+                     double *grn_load_bin, const CutReq *cut) {
   if (!c) return FALL3D_ADS_EINVAL;
   if (!o) return fail(c, FALL3D_ADS_EINVAL, "null argument");
+  if (cut) {
+    const fall3d_cut_args *ca = cut->args;
+    if (!ca) return fail(c, FALL3D_ADS_EINVAL, "null cut arguments");
+    if (!o->z_c || !ca->sigma) return fail(c, FALL3D_ADS_EINVAL, "cuts need the corner heights z_c and sigma levels");
+    const int ncut[4] = {ca->ncutx, ca->ncuty, ca->ncutz, ca->nfl};
+    const double *pos[4] = {ca->x_cut, ca->y_cut, ca->z_cut, ca->fl_cut};
+    for (int q = 0; q < 4; ++q)
+      if (ncut[q] < 0 || (ncut[q] > 0 && (!pos[q] || !cut->out[q])))
+        return fail(c, FALL3D_ADS_EINVAL, "cut kind %d: positive count needs positions and an output", q);
+  }
   if (!c->grid_ok) return fail(c, FALL3D_ADS_ESTATE, "fall3d_ads_set_grid has not been called");
   if (o->npm < 0 || o->npm > kMaxPm) return fail(c, FALL3D_ADS_EINVAL, "at most %d PM thresholds", kMaxPm);
   if ((col_mass || col_mass_pm || cloud_top) && !o->z_c)
@@ -212,7 +303,9 @@ extern "C" int fall3d_ads_postprocess(fall3d_ads_ctx *c, const fall3d_output_arg
                {cloud_top, ncol, nullptr},   {grn_load, ncol, nullptr},     {grn_load_bin, ncol * nb, nullptr}};
   bool ok = d_hm1 && d_hm2 && d_hm3 && d_spe && d_pmb && (!o->z_c || d_zc);
   for (auto &x : outs)
-    if (x.host && ok) ok = (x.dev = (double *)dev(8 * x.n)) != nullptr;
+    if ((x.host || (cut && &x == &outs[1])) && ok) ok = (x.dev = (double *)dev(8 * x.n)) != nullptr;
+  double *d_cz = nullptr, *d_zl = nullptr;
+  if (cut && ok) ok = (d_cz = (double *)dev(8 * n3)) != nullptr && (d_zl = (double *)dev(8 * (size_t)(nz + 1))) != nullptr;
   if (!ok) {
     cleanup();
     return fail(c, FALL3D_ADS_EHIP, "postprocess: device allocation failed");
@@ -247,6 +340,54 @@ extern "C" int fall3d_ads_postprocess(fall3d_ads_ctx *c, const fall3d_output_arg
     hipLaunchKernelGGL(k_output, dim3((unsigned)std::max<size_t>(g, 1)), dim3(256), 0, c->stream, a);
     e = hipGetLastError();
   }
+  // cuts: the species' concentration on the z levels sigma X3max, then
+  // linear interpolation to the cut positions (index and weight of each cut
+  // from grid_get_shapez over the corner coordinates, on the host)
+  std::vector<int> cidx[4];
+  std::vector<double> csh[4], zl;
+  if (cut && e == hipSuccess) {
+    const fall3d_cut_args *ca = cut->args;
+    zl.resize(nz + 1);
+    for (int k = 0; k <= nz; ++k) zl[k] = ca->sigma[k] * ca->X3max;
+    up(d_zl, zl.data(), 8 * zl.size());
+    if (e == hipSuccess) {
+      const size_t g = std::min<size_t>((ncol + 255) / 256, 65536);
+      hipLaunchKernelGGL(k_zlevels, dim3((unsigned)std::max<size_t>(g, 1)), dim3(256), 0, c->stream, a.con, d_zc, d_zl,
+                         nx, ny, nz, d_cz);
+      e = hipGetLastError();
+    }
+    std::vector<double> xc(nx + 1), yc(ny + 1);
+    for (int i = 0; i <= nx; ++i) xc[i] = ca->lonmin + (double)i * ca->dlon;
+    for (int j = 0; j <= ny; ++j) yc[j] = ca->latmin + (double)j * ca->dlat;
+    const int ncut[4] = {ca->ncutx, ca->ncuty, ca->ncutz, ca->nfl};
+    const double *pos[4] = {ca->x_cut, ca->y_cut, ca->z_cut, ca->fl_cut};
+    const std::vector<double> *coord[4] = {&xc, &yc, &zl, &zl};
+    for (int q = 0; q < 4 && e == hipSuccess; ++q) {
+      if (ncut[q] == 0) continue;
+      cidx[q].resize(ncut[q]);
+      csh[q].resize(ncut[q]);
+      for (int m = 0; m < ncut[q]; ++m) {
+        const double z = q == 3 ? pos[q][m] * 30.48 : pos[q][m];  // flight level (100 ft) -> m
+        shapez(coord[q]->data(), 1, (int)coord[q]->size(), z, cidx[q][m], csh[q][m]);
+      }
+      const int mode = q < 2 ? q : 2;
+      const size_t nout = (size_t)(mode == 0 ? ny + 1 : nx + 1) * (mode == 2 ? ny + 1 : nz + 1) * ncut[q];
+      int *d_idx = (int *)dev(4 * (size_t)ncut[q]);
+      double *d_sh = (double *)dev(8 * (size_t)ncut[q]), *d_out = (double *)dev(8 * nout);
+      if (!d_idx || !d_sh || !d_out) {
+        e = hipErrorOutOfMemory;
+        break;
+      }
+      up(d_idx, cidx[q].data(), 4 * (size_t)ncut[q]);
+      up(d_sh, csh[q].data(), 8 * (size_t)ncut[q]);
+      if (e != hipSuccess) break;
+      const size_t g = std::min<size_t>((nout + 255) / 256, 65536);
+      hipLaunchKernelGGL(k_cuts, dim3((unsigned)std::max<size_t>(g, 1)), dim3(256), 0, c->stream, d_cz, nx, ny, nz,
+                         mode, ncut[q], d_idx, d_sh, d_out);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(cut->out[q], d_out, 8 * nout, hipMemcpyDeviceToHost, c->stream);
+    }
+  }
   for (auto &x : outs)
     if (x.host && e == hipSuccess) e = hipMemcpyAsync(x.host, x.dev, 8 * x.n, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -255,4 +396,20 @@ extern "C" int fall3d_ads_postprocess(fall3d_ads_ctx *c, const fall3d_output_arg
   cleanup();
   if (e != hipSuccess) return fail(c, FALL3D_ADS_EHIP, "postprocess: %s", hipGetErrorString(e));
   return FALL3D_ADS_OK;
+}
+
+}  // namespace
+
+extern "C" int fall3d_ads_postprocess(fall3d_ads_ctx *c, const fall3d_output_args *o, double *con_bins, double *con,
+                                      double *con_sur, double *col_mass, double *col_mass_pm, double *cloud_top,
+                                      double *grn_load, double *grn_load_bin) {
+  return postprocess_impl(c, o, con_bins, con, con_sur, col_mass, col_mass_pm, cloud_top, grn_load, grn_load_bin,
+                          nullptr);
+}
+
+extern "C" int fall3d_ads_postprocess_cuts(fall3d_ads_ctx *c, const fall3d_output_args *o,
+                                           const fall3d_cut_args *cuts, double *cut_x, double *cut_y, double *cut_z,
+                                           double *cut_fl) {
+  const CutReq r = {cuts, {cut_x, cut_y, cut_z, cut_fl}};
+  return postprocess_impl(c, o, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &r);
 }

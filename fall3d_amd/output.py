@@ -7,13 +7,15 @@ computes every figure; this module only formats them as the reference's
 Fortran edit descriptors do (Ew.d, Fw.d, Iw).
 
 result_variables: the post-processed result fields of nc_IO_out_res for one
-species (fall3d_ads_postprocess), keyed by the reference's variable names.
+species (fall3d_ads_postprocess), keyed by the reference's variable names;
+cut_variables: its concentration cuts (fall3d_ads_postprocess_cuts), named
+as Sources/mod_nc_IO_names.F90:43-46 names them.
 """
 from __future__ import annotations
 
 import math
 
-__all__ = ["fortran_e", "fortran_f", "fortran_i", "budget_log", "should_log", "result_variables"]
+__all__ = ["fortran_e", "fortran_f", "fortran_i", "budget_log", "should_log", "result_variables", "cut_variables"]
 
 
 def fortran_e(x: float, w: int = 13, d: int = 6) -> str:
@@ -98,4 +100,14 @@ def result_variables(ads_ctx, species_name: str = "tephra", **kw) -> dict:
     names = {"con": "con", "con_bins": "con_bin", "con_sur": "sur", "col_mass": "col_mass",
              "col_mass_pm": "col_mass_pm", "cloud_top": "cloud_top", "grn_load": "grn_load",
              "grn_load_bin": "grn_load_bin"}
+    return {f"{species_name}_{names[k]}": v for k, v in out.items()}
+
+
+def cut_variables(ads_ctx, z_c, sigma, X3max, species_name: str = "tephra", **kw) -> dict:
+    """nc_IO_out_res's concentration cuts of one species from the device
+    state: <species>_con_yz (x cuts), _con_xz (y cuts), _con_xy (z cuts) and
+    _fl (flight-level cuts), g/m3. Keyword arguments go to
+    ADS.postprocess_cuts."""
+    out = ads_ctx.postprocess_cuts(z_c, sigma, X3max, **kw)
+    names = {"cut_x": "con_yz", "cut_y": "con_xz", "cut_z": "con_xy", "cut_fl": "fl"}
     return {f"{species_name}_{names[k]}": v for k, v in out.items()}

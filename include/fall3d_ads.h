@@ -275,6 +275,29 @@ int fall3d_ads_postprocess(fall3d_ads_ctx *ctx, const fall3d_output_args *args, 
                            double *con_sur, double *col_mass, double *col_mass_pm, double *cloud_top,
                            double *grn_load, double *grn_load_bin);
 
+/* Concentration cuts of nc_IO_out_res (Sources/mod_nc_IO.F90:3292-3393
+ * and 4084-4332) for the species of args: the species' corner
+ * concentration (CON, g/m3) is interpolated per corner column from the
+ * column's heights z_c onto the levels sigma(k) X3max, then linearly to the
+ * cut positions; each position's interval and weight are those of
+ * grid_get_shapez (Sources/mod_Grid.F90:1287-1329), so a position outside
+ * the mesh takes the end value (the reference then only warns). */
+typedef struct fall3d_cut_args {
+  double lonmin, dlon;   /* corner i (1..nx+1) at lonmin + (i-1) dlon (MY_GRID) */
+  double latmin, dlat;   /* corner j (1..ny+1) at latmin + (j-1) dlat */
+  const double *sigma;   /* gl_sigma(1:nz+1): the corner sigma levels */
+  double X3max;          /* domain top (m) */
+  int ncutx, ncuty, ncutz, nfl;
+  const double *x_cut, *y_cut, *z_cut; /* cut positions (x, y in the grid's units; z in m) */
+  const double *fl_cut;                /* flight levels (hundreds of feet: z = 30.48 FL m) */
+} fall3d_cut_args;
+
+/* Outputs (FP64 host arrays, a count of 0 leaves its array untouched and it
+ * may be NULL): cut_x (ny+1,nz+1,ncutx), cut_y (nx+1,nz+1,ncuty), cut_z
+ * (nx+1,ny+1,ncutz), cut_fl (nx+1,ny+1,nfl), first index fastest. */
+int fall3d_ads_postprocess_cuts(fall3d_ads_ctx *ctx, const fall3d_output_args *args, const fall3d_cut_args *cuts,
+                                double *cut_x, double *cut_y, double *cut_z, double *cut_fl);
+
 /* Dense source rate s(1:nx,1:ny,1:nz,nbins) (MY_TRA%my_s), injected before
  * the sweeps as c += dt s / (dX1_p dX2_p dX3_p Hm1 Hm2 Hm3)
  * (Sources/mod_F3D.F90:1456-1473); stored sparsely on the device (only

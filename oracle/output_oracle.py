@@ -11,7 +11,10 @@ their exact corner values, negative values clip to 0, map factors divide,
 the trapezoid column load of a profile linear in height is exact, the cloud
 top is the highest level over the threshold, PM selection by diameter, the
 Dobson conversion factor, and the ground-load corner averages of a field
-with clamped edges.
+with clamped edges; for the cuts, grid_get_shapez's intervals, weights and
+out-of-mesh ends, the exact z-level values of a field linear in height over
+terrain-following columns, and x, y, z and flight-level cuts of fields
+linear in x, y and z.
 """
 from __future__ import annotations
 
@@ -100,4 +103,51 @@ def postprocess(c, acum, z_c=None, hm1=None, hm2=None, hm3=None, bin_species=Non
                     acc = acc + 0.5 * (wp[:, :, k] + wp[:, :, k + 1]) * (z[:, :, k + 1] - z[:, :, k])
                 pm[..., p] = acc
         out["col_mass_pm"] = pm
+    return out
+
+
+def shapez(v, z):
+    """grid_get_shapez (mod_Grid.F90:1287-1329) over v(1..n): (ik, shape)
+    of the first interval [v(k), v(k+1)] holding z, searched upward; below
+    v(1) it is (1, 0), above v(n) (n-1, 1)."""
+    n = len(v)
+    if z < v[0]:
+        return 1, 0.0
+    if z > v[n - 1]:
+        return n - 1, 1.0
+    for k in range(1, n):
+        if v[k - 1] <= z <= v[k]:
+            return k, (z - v[k - 1]) / (v[k] - v[k - 1])
+    raise ValueError("levels not ordered upward")
+
+
+def cuts(con, z_c, sigma, X3max, lonmin=0.0, dlon=1.0, latmin=0.0, dlat=1.0, x_cut=(), y_cut=(), z_cut=(),
+         fl_cut=()):
+    """Concentration cuts of nc_IO_out_res (mod_nc_IO.F90:3292-3393 and
+    4084-4332) from the species' corner concentration con (nx+1, ny+1, nz+1):
+    con on the z levels sigma(k) X3max per corner column (interpolated over
+    the column's heights z_c), then interpolated to the cut positions."""
+    nbx, nby, nbz = con.shape
+    zl = [float(sigma[k]) * X3max for k in range(nbz)]
+    cz = np.zeros_like(con)
+    for j in range(nby):
+        for i in range(nbx):
+            col = [float(z) for z in z_c[i, j, :]]
+            for k in range(nbz):
+                kk, s = shapez(col, zl[k])
+                cz[i, j, k] = (1.0 - s) * con[i, j, kk - 1] + s * con[i, j, kk] if kk > 0 else 0.0
+    xc = [lonmin + float(i) * dlon for i in range(nbx)]
+    yc = [latmin + float(j) * dlat for j in range(nby)]
+    out = {"cut_x": np.zeros((nby, nbz, len(x_cut))), "cut_y": np.zeros((nbx, nbz, len(y_cut))),
+           "cut_z": np.zeros((nbx, nby, len(z_cut))), "cut_fl": np.zeros((nbx, nby, len(fl_cut)))}
+    for q, x in enumerate(x_cut):
+        ii, s = shapez(xc, x)
+        out["cut_x"][:, :, q] = (1.0 - s) * cz[ii - 1, :, :] + s * cz[ii, :, :]
+    for q, y in enumerate(y_cut):
+        jj, s = shapez(yc, y)
+        out["cut_y"][:, :, q] = (1.0 - s) * cz[:, jj - 1, :] + s * cz[:, jj, :]
+    for key, zs in (("cut_z", list(z_cut)), ("cut_fl", [f * 30.48 for f in fl_cut])):
+        for q, z in enumerate(zs):
+            kk, s = shapez(zl, z)
+            out[key][:, :, q] = (1.0 - s) * cz[:, :, kk - 1] + s * cz[:, :, kk]
     return out

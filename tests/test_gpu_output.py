@@ -58,3 +58,34 @@ def test_postprocess_matches_restatement(state):
     assert "tephra_col_mass" in names and names["tephra_grn_load"].shape == (nx + 1, ny + 1)
     print(budget_log(a.budget(), 12, dt, 12 * dt, 0.0, 100 * dt, "t", a.budget()["injected"]), flush=True)
     a.release()
+
+
+@pytest.mark.parametrize("state", ["fp32", "fp64"])
+def test_cuts_match_restatement(state):
+    print(f"case postprocess cuts {state}", flush=True)
+    p = operational_problem(40, 30, 14, 4, seed=9)
+    nx, ny, nz, nb = p.nx, p.ny, p.nz, p.nb
+    a = ads.ADS(nx, ny, nz, nb, "minmod", "rk", 8, state)
+    a.set_problem(p)
+    dt = a.time_step(0.0, 2, 0.9)
+    a.run(dt, 10, 0)
+    c, acum = a.get_state()
+    zc = _zc(nx, ny, nz, 11)
+    X3max = float(zc[:, :, -1].max())
+    sigma = np.linspace(0.0, 1.0, nz + 1) ** 1.3
+    spe = np.array([0, 1, 0, 0], dtype=np.int32)
+    kw = dict(lonmin=-3.0, dlon=0.5, latmin=10.0, dlat=0.25, x_cut=[-2.9, 0.0, 7.75, 99.0], y_cut=[10.1, 13.2],
+              z_cut=[50.0, 1500.0, 4000.0, 1e6], fl_cut=[20.0, 100.0, 330.0])
+    got = a.postprocess_cuts(zc, sigma, X3max, bin_species=spe, species=0, **kw)
+    con = O.postprocess(c, acum, z_c=zc, bin_species=spe, species=0)["con"]
+    ref = O.cuts(con, zc, sigma, X3max, **kw)
+    for k in ref:
+        assert got[k].shape == ref[k].shape, k
+        diff = np.count_nonzero(got[k] != ref[k])
+        if diff:
+            print(f"  {k}: {diff} differ, max {np.abs(got[k] - ref[k]).max():.3e}", flush=True)
+        assert diff == 0, k
+    assert ref["cut_z"][:, :, 1].max() > 0.0 and ref["cut_fl"].max() > 0.0
+    none = a.postprocess_cuts(zc, sigma, X3max, x_cut=[0.5])  # the other kinds may be left out
+    assert none["cut_x"].shape == (ny + 1, nz + 1, 1) and none["cut_z"].size == 0
+    a.release()

@@ -94,3 +94,64 @@ def test_ground_load_corner_average_with_clamped_edges():
             want = (val(i, j) + val(i - 1, j) + val(i, j - 1) + val(i - 1, j - 1)) / 4.0
             assert g[i - 1, j - 1] == want
     np.testing.assert_array_equal(out["grn_load"], g + 1.0)
+
+
+def test_shapez_intervals_weights_and_ends():
+    v = [0.0, 10.0, 30.0, 60.0]
+    assert O.shapez(v, 5.0) == (1, 0.5)
+    assert O.shapez(v, 10.0) == (1, 1.0)  # a node value: the first interval holding it, searched upward
+    assert O.shapez(v, 20.0) == (2, 0.5)
+    assert O.shapez(v, 45.0) == (3, 0.5)
+    assert O.shapez(v, 0.0) == (1, 0.0)
+    assert O.shapez(v, -1.0) == (1, 0.0)  # below the mesh: the bottom value
+    assert O.shapez(v, 61.0) == (3, 1.0)  # above: the top value
+
+
+def _terrain(nx, ny, nz, X3max=6000.0, seed=2):
+    rng = np.random.default_rng(seed)
+    h = rng.uniform(0.0, 900.0, (nx + 1, ny + 1))
+    sigma = np.concatenate([[0.0], np.cumsum(rng.uniform(1.0, 2.0, nz))])
+    sigma /= sigma[-1]
+    zc = h[:, :, None] + sigma[None, None, :] * (X3max - h[:, :, None])  # terrain-following corner heights
+    return sigma, zc, h
+
+
+def test_z_levels_are_exact_for_a_field_linear_in_height():
+    nx, ny, nz, X3max = 5, 4, 7, 6000.0
+    sigma, zc, h = _terrain(nx, ny, nz, X3max)
+    con = 3.0 + 2e-3 * zc  # linear in height a.s.l.
+    out = O.cuts(con, zc, sigma, X3max, z_cut=sigma * X3max)
+    cz = out["cut_z"]  # cut k at level k is that level of the z-level field
+    for i in range(nx + 1):
+        for j in range(ny + 1):
+            for k in range(nz + 1):
+                z = sigma[k] * X3max
+                want = 3.0 + 2e-3 * (z if z >= h[i, j] else h[i, j])  # under the terrain: the lowest corner
+                assert abs(cz[i, j, k] - want) <= 1e-13 * want, (i, j, k)
+
+
+def test_x_y_z_and_flight_level_cuts_of_linear_fields():
+    nx, ny, nz, X3max = 6, 5, 4, 8000.0
+    sigma = np.linspace(0.0, 1.0, nz + 1)
+    zc = np.broadcast_to(sigma * X3max, (nx + 1, ny + 1, nz + 1)).copy()  # flat terrain: z levels = corners
+    lonmin, dlon, latmin, dlat = -10.0, 0.5, 40.0, 0.25
+    i = np.arange(nx + 1)[:, None, None]
+    j = np.arange(ny + 1)[None, :, None]
+    x, y = lonmin + i * dlon, latmin + j * dlat
+    con = 1.0 + 0.3 * x + 0.7 * y + 1e-4 * zc
+    xs, ys, zs, fls = [-9.3, -7.25], [40.1, 41.0], [1234.0, 7999.0], [50.0, 200.0]
+    out = O.cuts(con, zc, sigma, X3max, lonmin, dlon, latmin, dlat, xs, ys, zs, fls)
+    xx, yy, zz = x[:, 0, 0], y[0, :, 0], sigma * X3max
+    for q, xv in enumerate(xs):
+        want = 1.0 + 0.3 * xv + 0.7 * yy[:, None] + 1e-4 * zz[None, :]
+        np.testing.assert_allclose(out["cut_x"][:, :, q], want, rtol=1e-13)
+    for q, yv in enumerate(ys):
+        want = 1.0 + 0.3 * xx[:, None] + 0.7 * yv + 1e-4 * zz[None, :]
+        np.testing.assert_allclose(out["cut_y"][:, :, q], want, rtol=1e-13)
+    for key, heights in (("cut_z", zs), ("cut_fl", [f * 30.48 for f in fls])):  # FL 50 = 1524 m, FL 200 = 6096 m
+        for q, zv in enumerate(heights):
+            want = 1.0 + 0.3 * xx[:, None] + 0.7 * yy[None, :] + 1e-4 * zv
+            np.testing.assert_allclose(out[key][:, :, q], want, rtol=1e-13)
+    # a cut outside the mesh takes the edge value
+    edge = O.cuts(con, zc, sigma, X3max, lonmin, dlon, latmin, dlat, x_cut=[lonmin - 5.0])["cut_x"][:, :, 0]
+    np.testing.assert_array_equal(edge, con[0, :, :])
