@@ -77,7 +77,7 @@ class _MeteoOptions(ctypes.Structure):
                 ("rams_cs", ctypes.c_double), ("dlon", ctypes.c_double), ("dlat", ctypes.c_double),
                 ("x3max", ctypes.c_double), ("s_time", ctypes.c_double), ("vset_fac", ctypes.c_double),
                 ("diam", ctypes.c_void_p), ("rho", ctypes.c_void_p), ("psi", ctypes.c_void_p),
-                ("cat", ctypes.c_void_p), ("aggregate", ctypes.c_void_p)]
+                ("cat", ctypes.c_void_p), ("aggregate", ctypes.c_void_p), ("dry_deposition", ctypes.c_int)]
 
 
 class _MeteoGrid(ctypes.Structure):
@@ -86,7 +86,8 @@ class _MeteoGrid(ctypes.Structure):
 
 
 class _MeteoCorner(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p * 2) for n in ("uc", "vc", "wc", "tc", "tvc", "rhoc", "pblhc", "ustc", "monc")]
+    _fields_ = [(n, ctypes.c_void_p * 2) for n in ("uc", "vc", "wc", "tc", "tvc", "rhoc", "pblhc", "ustc", "monc")] + \
+        [("lusec", ctypes.c_void_p)]
 
 
 class _OutputArgs(ctypes.Structure):
@@ -417,12 +418,14 @@ class ADS:
 
         corner: dict of the corner fields, each a pair (first time, second
         time): "uc", "vc", "wc", "tc", "rhoc" of shape (nx+1, ny+1, nz+1) and,
-        for similarity, "tvc" and the 2-D "pblhc", "ustc", "monc" (nx+1, ny+1).
+        for similarity, "tvc" and the 2-D "pblhc", "ustc", "monc" (nx+1, ny+1);
+        for dry deposition "ustc" and the land use "lusec" (one 2-D array).
         grid: "Hm1_p" (ny), "Hm2_p" (ny+4), "Hm3_p" (nx, ny), and as the
         options need them "dhdx_p", "dhdy_p", "z_c", "h_c", "X3_c", "X3_p".
         opts: "map_h", "map_v", "modkh", "modkv", "modv" (codes or names of
         METEO_*), "kh0", "kv0", "rams_cs", "dlon", "dlat", "x3max", "s_time",
-        "vset_fac" and the per-bin "diam", "rho", "psi", "cat", "aggregate"."""
+        "vset_fac", the per-bin "diam", "rho", "psi", "cat", "aggregate" and
+        "dry_deposition" (0 or 1)."""
         keep = []
 
         def arr(x, shape, name):
@@ -441,6 +444,7 @@ class ADS:
             if pair is not None:
                 getattr(cm, key)[0] = arr(pair[0], shp, key)
                 getattr(cm, key)[1] = arr(pair[1], shp, key)
+        cm.lusec = arr(corner.get("lusec"), c2, "lusec")
         gm = _MeteoGrid()
         for key, shp in (("Hm1_p", (ny,)), ("Hm2_p", (ny + 4,)), ("Hm3_p", (nx, ny)), ("dhdx_p", (nx, ny)),
                          ("dhdy_p", (nx, ny)), ("z_c", c3), ("h_c", c2), ("X3_c", (nz + 1,)), ("X3_p", (nz,))):
@@ -457,7 +461,8 @@ class ADS:
                            float(opts.get("kv0", 0.0)), float(opts.get("rams_cs", 0.2)), float(opts.get("dlon", 1.0)),
                            float(opts.get("dlat", 1.0)), float(opts.get("x3max", 0.0)), float(opts.get("s_time", 0.5)),
                            float(opts.get("vset_fac", 1.0)), bins["diam"].ctypes.data, bins["rho"].ctypes.data,
-                           bins["psi"].ctypes.data, bins["cat"].ctypes.data, bins["aggregate"].ctypes.data)
+                           bins["psi"].ctypes.data, bins["cat"].ctypes.data, bins["aggregate"].ctypes.data,
+                           int(opts.get("dry_deposition", 0)))
         self._chk(load_library().fall3d_ads_update_meteo(self._h, ctypes.byref(cm), ctypes.byref(gm),
                                                          ctypes.byref(om)), "update_meteo")
         del keep

@@ -17,6 +17,8 @@ struct MeteoArgs {
   const double *tc[2], *tvc[2], *rhoc[2];    // temperature, virtual temperature, air density
   const double *pblhc[2], *ustc[2], *monc[2];  // boundary-layer height, u*, Monin-Obukhov length
   const double *zc, *hc;                     // corner heights, corner topography
+  const double *lusec;                       // land use at the corners (dry deposition)
+  int drydep;                                // 1: add the dry deposition velocity at faces 0 and 1
   const double *hm1;                         // Hm1_p(1:ny)
   const double *hm2;                         // Hm2_p(-1:ny+2)
   const double *hm3, *dhdx, *dhdy;           // (nx, ny)

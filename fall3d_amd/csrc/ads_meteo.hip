@@ -332,6 +332,60 @@ __device__ bool vset_point(double diam, double rhop, double psi, double rhoa, do
 // v_s at (i, j, f = 0..nz+2, bin): air density and temperature at the
 // w-boundary from the time-interpolated corners, Sutherland viscosity,
 // aggregates scaled by vset_fac, / Hm3; ghosts copy faces 1 and nz+1
+// Dry deposition velocity (Feng 2008; phys_dry_deposition, Sources/mod_Phys.F90:
+// 1479-1673). Coefficients a, b per land class (urban, remote continental,
+// desert, polar, marine, rural) and aerosol mode (giant, coarse,
+// accumulation, nuclei), :1521-1571.
+__constant__ double kDryA[6][4] = {{0.0048, 0.0315, 1.2891, 1.0338}, {0.0037, 0.0120, 1.3977, 1.0707},
+                                   {0.0042, 0.2928, 1.3970, 0.9155}, {0.0032, 0.1201, 1.1838, 1.0096},
+                                   {0.0043, 0.1337, 1.2834, 1.1595}, {0.0045, 0.0925, 1.2654, 1.0891}};
+__constant__ double kDryB[6][4] = {{1.0, 2.7925, 2.6878, 1.2644}, {1.0, 2.2413, 2.5838, 1.3247},
+                                   {1.0, 3.8581, 2.5580, 1.0364}, {1.0, 3.4407, 2.8033, 1.2069},
+                                   {1.0, 3.5456, 2.7157, 1.4863}, {1.0, 3.2920, 2.7227, 1.4240}};
+
+// mode of a bin by diameter: 0 above 100 um (none), 1 giant, 2 coarse,
+// 3 accumulation, 4 nuclei (:1575-1587)
+__device__ __forceinline__ int drydep_mode(double d) {
+  return d > 100e-6 ? 0 : d > 10e-6 ? 1 : d > 2.5e-6 ? 2 : d > 0.1e-6 ? 3 : 4;
+}
+
+// Feng land class (1..6) of a USGS 24-category land use (:1592-1608)
+__device__ __forceinline__ int land_class(int cat) {
+  switch (cat) {
+    case 1: return 1;
+    case 10: case 19: return 3;
+    case 16: return 5;
+    case 20: case 21: case 22: case 23: return 2;
+    case 24: return 4;
+    default: return 6;
+  }
+}
+
+// deposition velocity of mass-point column (i, j) for a bin of the given
+// mode: series of the aerodynamic resistance 1/(k u*) and the surface term
+// 0.0226 u* exp(-(Re*-40300)^2 / (2 15330^2)) + a u*^b (:1613-1670)
+__device__ double drydep_velocity(const MeteoArgs &a, const Corner &C, int i, int j, int mode) {
+  const double s = a.s_time;
+  auto U = [&](int ii, int jj) { return lerp2(a.ustc[0], a.ustc[1], s, C.at2(ii, jj)); };
+  auto T = [&](int ii, int jj) { return lerp2(a.tc[0], a.tc[1], s, C.at(ii, jj, 1)); };
+  auto R = [&](int ii, int jj) { return lerp2(a.rhoc[0], a.rhoc[1], s, C.at(ii, jj, 1)); };
+  auto Z = [&](int ii, int jj, int k) { return a.zc[C.at(ii, jj, k)]; };
+  const double ust = 0.25 * (U(i, j) + U(i + 1, j) + U(i + 1, j + 1) + U(i, j + 1));
+  const double ra = ust > 0.0 ? 1.0 / (kKarman * ust) : 1e20;
+  const double t = 0.25 * (T(i, j) + T(i + 1, j) + T(i + 1, j + 1) + T(i, j + 1));
+  const double visc = kVisa0 * ((kTa0 + 120.0) / (t + 120.0)) * pow(t / kTa0, 1.5);
+  const double rho = 0.25 * (R(i, j) + R(i + 1, j) + R(i + 1, j + 1) + R(i, j + 1));
+  const double z1 = 0.25 * (Z(i, j, 1) + Z(i + 1, j, 1) + Z(i + 1, j + 1, 1) + Z(i, j + 1, 1));
+  const double z2 = 0.25 * (Z(i, j, 2) + Z(i + 1, j, 2) + Z(i + 1, j + 1, 2) + Z(i, j + 1, 2));
+  const double re = ust * (z2 - z1) * 0.5 * rho / visc;
+  double vd1 = -0.5 * ((re - 40300.0) / 15330.0) * ((re - 40300.0) / 15330.0);
+  vd1 = 0.0226 * ust * exp(vd1);
+  const int land = land_class((int)a.lusec[C.at2(i, j)]);  // INT(): the land use of corner (i, j)
+  double vd = vd1 + kDryA[land - 1][mode - 1] * pow(ust, kDryB[land - 1][mode - 1]);
+  vd = 1.0 / (ra + 1.0 / vd);
+  return vd;
+}
+
 __global__ void k_met_vs(MeteoArgs a) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const size_t n = (size_t)nx * ny * (nz + 3) * a.nb;
@@ -353,6 +407,10 @@ __global__ void k_met_vs(MeteoArgs a) {
       if (!vset_point(a.diam[b], a.rhop[b], a.psi[b], rhoa, visa, a.modv, v)) atomicOr(a.flag, 1u);
     }
     if (a.aggregate[b]) v = a.vset_fac * v;
+    if (a.drydep && f <= 1) {  // ground face and the one below it, before scaling
+      const int mode = drydep_mode(a.diam[b]);
+      if (mode > 0) v = v + drydep_velocity(a, C, i, j, mode);
+    }
     v = v / a.hm3[(size_t)(i - 1) + (size_t)nx * (j - 1)];
     put(a.vs32, a.vs64, q, v);
   }
@@ -505,6 +563,9 @@ int fall3d_ads_update_meteo(fall3d_ads_ctx *c, const fall3d_meteo_corner *m, con
   if (o->map_v == 2 && (!g->X3_c || !g->X3_p || !(o->x3max > 0.0)))
     return fail(c, FALL3D_ADS_EINVAL, "linear decay needs X3_c, X3_p and x3max > 0");
   if (sim && (!g->z_c || !g->h_c)) return fail(c, FALL3D_ADS_EINVAL, "similarity needs z_c and h_c");
+  const bool dd = o->dry_deposition != 0;
+  if (dd && (!m->ustc[0] || !m->ustc[1] || !m->lusec || !g->z_c))
+    return fail(c, FALL3D_ADS_EINVAL, "dry deposition needs ustc, lusec and z_c");
 
   const int nx = c->nx, ny = c->ny, nz = c->nz;
   const size_t n3 = (size_t)(nx + 1) * (ny + 1) * (nz + 1), n2 = (size_t)(nx + 1) * (ny + 1);
@@ -546,10 +607,12 @@ int fall3d_ads_update_meteo(fall3d_ads_ctx *c, const fall3d_meteo_corner *m, con
       a.tvc[t] = tmp;
       if ((rc = up64(c, d, m->pblhc[t], n2, raw, &tmp))) return rc;
       a.pblhc[t] = tmp;
-      if ((rc = up64(c, d, m->ustc[t], n2, raw, &tmp))) return rc;
-      a.ustc[t] = tmp;
       if ((rc = up64(c, d, m->monc[t], n2, raw, &tmp))) return rc;
       a.monc[t] = tmp;
+    }
+    if (sim || dd) {
+      if ((rc = up64(c, d, m->ustc[t], n2, raw, &tmp))) return rc;
+      a.ustc[t] = tmp;
     }
   }
   if ((rc = up64(c, d, g->Hm1_p, (size_t)ny, raw, &tmp))) return rc;
@@ -570,11 +633,18 @@ int fall3d_ads_update_meteo(fall3d_ads_ctx *c, const fall3d_meteo_corner *m, con
     if ((rc = up64(c, d, g->X3_p, (size_t)nz, raw, &tmp))) return rc;
     a.x3p = tmp;
   }
-  if (sim) {
+  if (sim || dd) {
     if ((rc = up64(c, d, g->z_c, n3, raw, &tmp))) return rc;
     a.zc = tmp;
+  }
+  if (sim) {
     if ((rc = up64(c, d, g->h_c, n2, raw, &tmp))) return rc;
     a.hc = tmp;
+  }
+  if (dd) {
+    if ((rc = up64(c, d, m->lusec, n2, raw, &tmp))) return rc;
+    a.lusec = tmp;
+    a.drydep = 1;
   }
   a.dx1p = c->dxp64[0];
   a.dx2p = c->dxp64[1];

@@ -421,15 +421,17 @@ def random_problem(nx, ny, nz, nb, seed=0, dx=1000.0, dy=1200.0, dz=150.0, wscal
 
 
 def meteo_corner_problem(nx, ny, nz, nb, seed=0, map_v=0, map_h=0, modkh=2, modkv=3, modv=2, dx=5000.0, dy=5000.0,
-                         ztop=12000.0, s_time=0.5):
+                         ztop=12000.0, s_time=0.5, dry_deposition=False, diam=None):
     """Synthetic inputs of the meteo update (F3D_update_meteo_term): corner
     fields at two meteo times (sheared winds with small-scale noise, a
     stratified temperature and density, boundary-layer height, u* and a
     Monin-Obukhov length of both signs, |L| < 10 included), terrain with
     slopes for the sigma maps, map factors away from 1, and nb particle
     bins (1 um - 2 mm, densities 800-2600 kg/m3, shape factors 0.5-1, one
-    aerosol bin, one aggregate bin). Returns (grid, corner, mgrid, opts):
-    grid the ADS spacing arrays, the rest the update's arguments."""
+    aerosol bin, one aggregate bin; diam overrides the diameters), and USGS
+    land-use codes 0-25 at the corners (unknown codes included) for dry
+    deposition. Returns (grid, corner, mgrid, opts): grid the ADS spacing
+    arrays, the rest the update's arguments."""
     rng = np.random.default_rng(seed)
     x3c = np.linspace(0.0, ztop, nz + 1)                       # sigma levels (corner heights above terrain)
     x3p = 0.5 * (x3c[1:] + x3c[:-1])
@@ -454,6 +456,7 @@ def meteo_corner_problem(nx, ny, nz, nb, seed=0, map_v=0, map_h=0, modkh=2, modk
         mon = rng.uniform(-400.0, 400.0, (nx + 1, ny + 1))
         mon[::3, ::4] = rng.uniform(-12.0, 12.0, mon[::3, ::4].shape)  # |L| < 10 clamps
         corner["monc"][t] = np.asfortranarray(mon)
+    corner["lusec"] = np.asfortranarray(rng.integers(0, 26, (nx + 1, ny + 1)).astype(np.float64))
     hm3 = 1.0 - c2p_2d_np(h_c) / ztop
     mgrid = {
         "Hm1_p": rng.uniform(0.8, 1.0, ny) if map_h else np.ones(ny),
@@ -464,7 +467,7 @@ def meteo_corner_problem(nx, ny, nz, nb, seed=0, map_v=0, map_h=0, modkh=2, modk
         "z_c": np.asfortranarray(z_c), "h_c": np.asfortranarray(h_c), "X3_c": x3c, "X3_p": x3p,
     }
     g = uniform_grid(nx, ny, nz, dx, dy, ztop / nz)
-    diam = np.geomspace(1e-6, 2e-3, nb)
+    diam = np.geomspace(1e-6, 2e-3, nb) if diam is None else np.asarray(diam, dtype=np.float64)
     cat = np.ones(nb, dtype=np.int32)
     agg = np.zeros(nb, dtype=np.int32)
     if nb > 2:
@@ -473,7 +476,7 @@ def meteo_corner_problem(nx, ny, nz, nb, seed=0, map_v=0, map_h=0, modkh=2, modk
     opts = {"map_h": map_h, "map_v": map_v, "modkh": modkh, "modkv": modkv, "modv": modv,
             "kh0": 250.0, "kv0": 15.0, "rams_cs": 0.2275, "dlon": dx, "dlat": dy, "x3max": ztop,
             "s_time": s_time, "vset_fac": 0.85, "diam": diam, "rho": rng.uniform(800.0, 2600.0, nb),
-            "psi": rng.uniform(0.5, 1.0, nb), "cat": cat, "aggregate": agg}
+            "psi": rng.uniform(0.5, 1.0, nb), "cat": cat, "aggregate": agg, "dry_deposition": int(dry_deposition)}
     return g, corner, mgrid, opts
 
 

@@ -206,6 +206,10 @@ typedef struct fall3d_meteo_options {
   const double *diam, *rho, *psi;  /* per bin: diameter, density, shape factor (MY_BIN) */
   const int *cat;                  /* per bin: category (2 = aerosol: v_s = 0) */
   const int *aggregate;            /* per bin: 1 for an aggregate bin */
+  int dry_deposition;              /* 1: add the dry deposition velocity (Feng 2008,
+                                    * Sources/mod_Phys.F90:1479-1673) to v_s at the ground
+                                    * face and the one below it, before scaling
+                                    * (Sources/mod_F3D.F90:790-801); needs ustc, lusec, z_c */
 } fall3d_meteo_options;
 
 /* Grid arrays of the update (host arrays of real_kind). */
@@ -227,6 +231,7 @@ typedef struct fall3d_meteo_corner {
   const void *uc[2], *vc[2], *wc[2];
   const void *tc[2], *tvc[2], *rhoc[2];
   const void *pblhc[2], *ustc[2], *monc[2];
+  const void *lusec; /* (1:nx+1,1:ny+1) USGS land use (MY_MET%my_lusec); read with dry_deposition */
 } fall3d_meteo_corner;
 
 /* Runs the update and makes its output the context's meteo (as set_meteo);

@@ -21,7 +21,9 @@ against hand-evaluated profiles; the drag laws against published sphere
 limits and Stokes' law, the iteration's fixed point against an independent
 root finder, Dellino's closed form; the metric scalings on Cartesian maps
 reduce to the identity and on sigma maps follow the hand-evaluated terrain
-correction.
+correction; the dry deposition velocity: mode thresholds, land classes and
+table orientation, resistances in series, the Re* Gaussian centred at 40300,
+zero u*, and only the two lowest faces of the bins up to 100 um changed.
 """
 from __future__ import annotations
 
@@ -260,6 +262,79 @@ def vs_field(opts, rhoc, tc, nb):
 
 
 # ---------------------------------------------------------------------------
+# Dry deposition velocity (Feng 2008), phys_dry_deposition, mod_Phys.F90:1479-1673
+# coefficients a, b of each land class (rows 1..6: urban, remote continental,
+# desert, polar, marine, rural) and aerosol mode (columns 1..4: giant, coarse,
+# accumulation, nuclei), mod_Phys.F90:1521-1571
+DRYDEP_A = np.array([[0.0048, 0.0315, 1.2891, 1.0338], [0.0037, 0.0120, 1.3977, 1.0707],
+                     [0.0042, 0.2928, 1.3970, 0.9155], [0.0032, 0.1201, 1.1838, 1.0096],
+                     [0.0043, 0.1337, 1.2834, 1.1595], [0.0045, 0.0925, 1.2654, 1.0891]])
+DRYDEP_B = np.array([[1.0, 2.7925, 2.6878, 1.2644], [1.0, 2.2413, 2.5838, 1.3247],
+                     [1.0, 3.8581, 2.5580, 1.0364], [1.0, 3.4407, 2.8033, 1.2069],
+                     [1.0, 3.5456, 2.7157, 1.4863], [1.0, 3.2920, 2.7227, 1.4240]])
+
+
+def drydep_mode(diam):
+    """Aerosol mode of a bin by diameter: 0 above 100 um (no deposition
+    velocity), 1 giant (> 10 um), 2 coarse (> 2.5 um), 3 accumulation
+    (> 0.1 um), 4 nuclei (mod_Phys.F90:1575-1587)."""
+    if diam > 100e-6:
+        return 0
+    if diam > 10e-6:
+        return 1
+    if diam > 2.5e-6:
+        return 2
+    if diam > 0.1e-6:
+        return 3
+    return 4
+
+
+def land_class(category):
+    """Feng land class of a USGS 24-category land use (mod_Phys.F90:1592-1608)."""
+    c = int(category)
+    if c == 1:
+        return 1
+    if c in (10, 19):
+        return 3
+    if c == 16:
+        return 5
+    if c in (20, 21, 22, 23):
+        return 2
+    if c == 24:
+        return 4
+    return 6
+
+
+def drydep_velocity(diam, ustc, tc, rhoc, zc, lusec):
+    """(nx, ny, nb) deposition velocity of each mass-point column and bin
+    (0 for bins above 100 um) from the time-interpolated corner u*, the
+    lowest corner temperature and density, the two lowest corner heights
+    and the land use at corner (i, j) (mod_Phys.F90:1613-1670)."""
+    avg = lambda f: 0.25 * (f[:-1, :-1] + f[1:, :-1] + f[1:, 1:] + f[:-1, 1:])
+    ust = avg(ustc)
+    with np.errstate(divide="ignore"):
+        ra = np.where(ust > 0.0, 1.0 / (KARMAN * np.where(ust > 0.0, ust, 1.0)), 1e20)
+    T = avg(tc[:, :, 0])
+    visc = VISA0 * ((TA0 + 120.0) / (T + 120.0)) * np.power(T / TA0, 1.5)
+    rho = avg(rhoc[:, :, 0])
+    z1, z2 = avg(zc[:, :, 0]), avg(zc[:, :, 1])
+    Re = ust * (z2 - z1) * 0.5 * rho / visc
+    vd1 = -0.5 * ((Re - 40300.0) / 15330.0) * ((Re - 40300.0) / 15330.0)
+    vd1 = 0.0226 * ust * np.exp(vd1)
+    land = np.vectorize(land_class)(np.asarray(lusec)[:-1, :-1]) - 1
+    out = np.zeros(ust.shape + (len(diam),))
+    for b, d in enumerate(diam):
+        m = drydep_mode(float(d))
+        if m == 0:
+            continue
+        a, bb = DRYDEP_A[land, m - 1], DRYDEP_B[land, m - 1]
+        vd = vd1 + a * np.power(ust, bb)
+        with np.errstate(divide="ignore"):
+            out[:, :, b] = 1.0 / (ra + 1.0 / vd)
+    return out
+
+
+# ---------------------------------------------------------------------------
 def update_meteo(nx, ny, nz, nb, dX1_p, dX2_p, corner, grid, opts):
     """The whole update; returns a dict of the ADS input fields (see the
     module docstring) and "converged"."""
@@ -344,6 +419,15 @@ def update_meteo(nx, ny, nz, nb, dX1_p, dX2_p, corner, grid, opts):
     vsg[:, :, 1:-1] = vs
     vsg[:, :, 0] = vs[:, :, 0]
     vsg[:, :, -1] = vs[:, :, -1]
+    if int(opts.get("dry_deposition", 0)):  # added at the ground face and the one below it, before scaling
+        ustl = (1.0 - s) * np.asarray(corner["ustc"][0], dtype=np.float64) + \
+            s * np.asarray(corner["ustc"][1], dtype=np.float64)
+        vd = drydep_velocity(opts["diam"], ustl, tcl, rhocl, np.asarray(grid["z_c"], dtype=np.float64),
+                             corner["lusec"])
+        for b in range(nb):
+            if drydep_mode(float(opts["diam"][b])) > 0:
+                vsg[:, :, 0, b] = vsg[:, :, 0, b] + vd[:, :, b]
+                vsg[:, :, 1, b] = vsg[:, :, 1, b] + vd[:, :, b]
     out["vs"] = _F(vsg / hm3[:, :, None, None])
     out["converged"] = ok
     return out

@@ -57,6 +57,37 @@ def test_update_meteo_matches_restatement(case, state):
     a.release()
 
 
+@pytest.mark.parametrize("state", ["fp64", "fp32"])
+@pytest.mark.parametrize("modv", [2, 6])
+def test_dry_deposition_matches_restatement(modv, state):
+    """Dry deposition velocities added at the two lowest faces (bins of all
+    four Feng modes, one above 100 um, an aerosol bin; land-use codes 0-25 on
+    a sigma map): the device's v_s against the restatement; exp and pow give
+    last-bit differences only, the drag law iterates to 1e-6."""
+    print(f"case meteo dry deposition modv={modv} {state}", flush=True)
+    nx, ny, nz, nb = 23, 17, 11, 5
+    g, corner, mg, opts = meteo_corner_problem(nx, ny, nz, nb, seed=21 + modv, map_v=1, modkh=2, modkv=1, modv=modv,
+                                               s_time=0.37, dry_deposition=True,
+                                               diam=[5e-8, 1e-6, 5e-6, 5e-5, 2e-4])
+    ref = M.update_meteo(nx, ny, nz, nb, g["dX1_p"], g["dX2_p"], corner, mg, opts)
+    base = M.update_meteo(nx, ny, nz, nb, g["dX1_p"], g["dX2_p"], corner, mg, dict(opts, dry_deposition=0))
+    assert ref["converged"] and np.all(ref["vs"][:, :, :2, :4] > base["vs"][:, :, :2, :4])
+    a = ads.ADS(nx, ny, nz, nb, "minmod", "rk", 8, state, grid=g)
+    a.update_meteo(corner, mg, opts)
+    got = a.get_meteo()["vs"]
+    e = _rel(got, ref["vs"])
+    print(f"  vs: max rel {e:.2e}", flush=True)
+    assert e <= 1e-5
+    # the deposition part alone (the settling part cancels), to FP64 rounding in the FP64 state
+    if state == "fp64":
+        a.update_meteo(corner, mg, dict(opts, dry_deposition=0))
+        got0 = a.get_meteo()["vs"]
+        e2 = _rel(got[:, :, :2, :4] - got0[:, :, :2, :4], ref["vs"][:, :, :2, :4] - base["vs"][:, :, :2, :4])
+        print(f"  deposition part: max rel {e2:.2e}", flush=True)
+        assert e2 <= 1e-12
+    a.release()
+
+
 def test_step_after_update_matches_oracle_on_the_same_fields():
     """The updated fields drive the step: with the FP64 state, steps on the
     device meteo equal the oracle's steps on the fields read back."""

@@ -225,3 +225,66 @@ def test_cartesian_scaling_is_identity_and_sigma_correction():
         np.testing.assert_allclose(w[f], (W - hx * U - hy * V) / h3, rtol=1e-13)
     # k3 on the sigma map: (kv + (hx^2 + hy^2) kh) / Hm3^2 with constant kh, kv
     np.testing.assert_allclose(out["k3"][2], (15.0 + hx * hx * 250.0 + hy * hy * 250.0) / (h3 * h3), rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# dry deposition (Feng 2008), oracle/meteo_oracle.py:drydep_velocity
+def test_drydep_modes_and_land_classes():
+    assert [M.drydep_mode(d) for d in (2e-4, 100.0001e-6, 100e-6, 10.0001e-6, 10e-6, 2.5e-6, 1e-7, 5e-8)] == \
+        [0, 0, 1, 1, 2, 3, 4, 4]  # strict ">" at every threshold
+    want = {1: 1, 10: 3, 19: 3, 16: 5, 20: 2, 21: 2, 22: 2, 23: 2, 24: 4}
+    for c in range(0, 26):
+        assert M.land_class(c) == want.get(c, 6), c  # everything else (and unknown codes) rural
+    assert M.land_class(16.9) == 5  # INT() truncates
+
+
+def _uniform_column(nx, ny, ust, T, rho, dz, land):
+    f2 = lambda v: np.full((nx + 1, ny + 1), float(v))
+    tc = np.full((nx + 1, ny + 1, 2), float(T))
+    rhoc = np.full((nx + 1, ny + 1, 2), float(rho))
+    zc = np.stack([f2(0.0), f2(dz)], axis=2)
+    return f2(ust), tc, rhoc, zc, f2(land)
+
+
+def test_drydep_velocity_structure():
+    # T = Ta0 makes the viscosity visa0 exactly; dz puts Re* at the Gaussian's centre 40300
+    ust, rho = 0.3, 1.1
+    dz = 2.0 * 40300.0 * M.VISA0 / (ust * rho)
+    diam = [5e-5, 5e-6, 1e-6, 5e-8, 2e-4]  # modes 1, 2, 3, 4 and none
+    for land_code, land in ((10, 3), (16, 5), (1, 1)):
+        u, tc, rhoc, zc, lu = _uniform_column(3, 2, ust, M.TA0, rho, dz, land_code)
+        vd = M.drydep_velocity(diam, u, tc, rhoc, zc, lu)
+        ra = 1.0 / (0.4 * ust)
+        for b, mode in enumerate((1, 2, 3, 4)):
+            # surface term from the table row of the land class and column of the mode
+            vs = 0.0226 * ust + M.DRYDEP_A[land - 1, mode - 1] * ust ** M.DRYDEP_B[land - 1, mode - 1]
+            v = vd[:, :, b]
+            assert np.all(v < 1.0 / ra) and np.all(v < vs)               # resistances in series
+            np.testing.assert_allclose(1.0 / v, ra + 1.0 / vs, rtol=1e-12)
+        assert np.all(vd[:, :, 4] == 0.0)                                # > 100 um: none
+    # table orientation: desert / coarse is the steepest entry of its row
+    assert (M.DRYDEP_A[2, 1], M.DRYDEP_B[2, 1]) == (0.2928, 3.8581)
+    # the Re* term is a Gaussian centred at 40300: equal at 40300 -+ delta, largest at the centre
+    vals = []
+    for f in (0.7, 1.0, 1.3):
+        u, tc, rhoc, zc, lu = _uniform_column(2, 2, ust, M.TA0, rho, dz * f, 1)
+        vals.append(M.drydep_velocity([5e-5], u, tc, rhoc, zc, lu)[0, 0, 0])
+    assert vals[1] > vals[0] and vals[1] > vals[2] and abs(vals[0] - vals[2]) <= 1e-12 * vals[1]
+    # no friction velocity, no deposition velocity
+    u, tc, rhoc, zc, lu = _uniform_column(2, 2, 0.0, 280.0, rho, 50.0, 7)
+    assert np.all(M.drydep_velocity([1e-6], u, tc, rhoc, zc, lu) == 0.0)
+
+
+def test_drydep_enters_the_two_lowest_faces_only():
+    nx, ny, nz = 6, 5, 4
+    diam = np.array([5e-8, 1e-6, 5e-6, 5e-5, 2e-4])
+    g, corner, mg, opts = meteo_corner_problem(nx, ny, nz, 5, seed=4, map_v=1, modkv=1, modv=2)
+    opts = dict(opts, diam=diam)
+    corner = dict(corner, lusec=np.asfortranarray(np.random.default_rng(1).integers(0, 26, (nx + 1, ny + 1)) * 1.0))
+    base = M.update_meteo(nx, ny, nz, 5, g["dX1_p"], g["dX2_p"], corner, mg, opts)["vs"]
+    dd = M.update_meteo(nx, ny, nz, 5, g["dX1_p"], g["dX2_p"], corner, mg, dict(opts, dry_deposition=1))["vs"]
+    assert np.array_equal(dd[:, :, 2:, :], base[:, :, 2:, :])   # faces above the ground untouched
+    assert np.array_equal(dd[..., 4], base[..., 4])              # the 200 um bin has no deposition velocity
+    add = dd[:, :, :2, :4] - base[:, :, :2, :4]
+    assert np.all(add > 0.0) and np.allclose(add[:, :, 0], add[:, :, 1], rtol=1e-12)
+    assert np.all(dd[:, :, :2, 0] > 0.0) and np.all(base[:, :, :, 0] == 0.0)  # the aerosol bin moves too
