@@ -61,7 +61,9 @@ def parse():
                          "fixed tile shapes instead of the autotuner (profiling runs)")
     ap.add_argument("--eval-steps", type=int, default=100,
                     help="learned workload: coarse steps of the accuracy evaluation (fine: twice as many)")
-    ap.add_argument("--train-seeds", type=int, default=8, help="learned workload: training seeds (512x512x40x4)")
+    ap.add_argument("--train-seeds", type=int, default=32, help="learned workload: training seeds (512x512x40x4)")
+    ap.add_argument("--train-steps", type=int, default=2000, help="learned workload: Adam steps of 65536 rows")
+    ap.add_argument("--gains", default="1.0,0.5", help="learned workload: correction gains evaluated")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB with torch and with the library's "
                          "16-byte buffer-load copy (calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -376,7 +378,7 @@ def learned_compare(args):
     net = ads.Network(capacity=int(args.train_seeds * 8_000_000 + 1_000_000))
     rows = learned.collect(net, range(args.train_seeds), tr, nwarm=40, nsamples=8, gap=60, **kw_tr)
     t1 = time.perf_counter()
-    loss = net.train(2000, batch=65536, lr=1e-3, seed=1)
+    loss = net.train(args.train_steps, batch=65536, lr=1e-3, seed=1)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     log(f"learned: {rows} rows in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s")
@@ -420,7 +422,9 @@ def learned_compare(args):
         f"dt {dt_c:.2f} s -> {speedup:.2f}x at equal simulated time")
     del pf, pc
     res = learned.evaluate(net, LARGE_SEED, LARGE, args.eval_steps, smin=1e-10, truth_state="fp64",
-                           fine_fp32=True, gains=(1.0, 0.5), dtype=f32, real_kind=4, **kw)
+                           fine_fp32=True, gains=tuple(float(g) for g in args.gains.split(",")), dtype=f32,
+                           real_kind=4,
+                           progress=lambda m: log(f"learned: {m}"), **kw)
     net.release()
     log(f"learned: evaluated {res}")
     return {
@@ -431,7 +435,7 @@ def learned_compare(args):
                    "dt_fine_s": dt_f, "dt_coarse_s": dt_c, "fine_steps_timed": kf, "fine_ms_per_step":
                    round(fine_ms, 4), "fine_plain_gcbs": round(thr_fine, 3),
                    "speedup_equal_sim_time": round(speedup, 3),
-                   "training": {"grid": list(tr), "seeds": args.train_seeds, "rows": rows,
+                   "training": {"grid": list(tr), "seeds": args.train_seeds, "rows": rows, "adam_steps": args.train_steps,
                                 "collect_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 2),
                                 "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-50:].mean())},
                    "eval_coarse_steps": args.eval_steps, "errors_vs_fine_fp64": res,

@@ -80,8 +80,24 @@ def rel_l1(a, ref):
     return float(np.abs(a - ref).sum() / max(np.abs(ref).sum(), 1e-300))
 
 
+def _run_reporting(g, dt, nsteps, progress, what, chunk=200):
+    """g.run(dt, nsteps, 0) in chunks of an even number of steps (the x/y
+    order follows the iteration count), calling progress(message) after each
+    chunk when given."""
+    if progress is None:
+        g.run(dt, nsteps, 0)
+        return
+    done = 0
+    while done < nsteps:
+        n = min(chunk, nsteps - done)
+        g.run(dt, n, done)
+        g.synchronize()
+        done += n
+        progress(f"{what}: {done}/{nsteps} steps")
+
+
 def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", gain=1.0, fine_fp32=False,
-             gains=None, variants=None, real_kind=8, dtype=None, **kw):
+             gains=None, variants=None, real_kind=8, dtype=None, progress=None, **kw):
     """Run fine (2 nsteps steps), coarse plain and coarse learned (nsteps steps
     each) from the same initial state; compare the coarse column loads and
     ground deposits with the fine ones averaged onto the coarse grid.
@@ -90,14 +106,15 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", g
     correction gains at once (keys "learned_<gain>"); variants: name ->
     keyword arguments of ADS.set_learned (smin, gain, flux_limit).
     real_kind 4 (dtype float32) keeps the host arrays of large grids in
-    float32. Returns a dict of relative L1 errors."""
+    float32. progress: optional callable given a line of text every few
+    hundred steps of each run. Returns a dict of relative L1 errors."""
     nx, ny, nz, nb = size
     dtype = dtype or (np.float32 if real_kind == 4 else np.float64)
     fine, coarse = make_pair(nx, ny, nz, nb, seed, state=truth_state, real_kind=real_kind, **kw)
     dt = coarse_dt(fine, coarse, cfl)
     coarse.release()
     dz = kw.get("dz", 500.0)
-    fine.run(0.5 * dt, 2 * nsteps, 0)
+    _run_reporting(fine, 0.5 * dt, 2 * nsteps, progress, f"fine {truth_state}")
     cf, af = fine.get_state()
     fine.release()
     lf = _pairs_mean(column_load(cf, nx, ny, nz, dz))
@@ -107,7 +124,7 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", g
         g = make_pair(nx, ny, nz, nb, seed, state="fp32", real_kind=real_kind, **kw)
         g[1].release()
         g = g[0]
-        g.run(0.5 * dt, 2 * nsteps, 0)
+        _run_reporting(g, 0.5 * dt, 2 * nsteps, progress, "fine fp32")
         c2, a2 = g.get_state()
         g.release()
         out["fine_fp32"] = {"load_l1": rel_l1(_pairs_mean(column_load(c2, nx, ny, nz, dz)), lf),
@@ -127,7 +144,7 @@ def evaluate(net, seed, size, nsteps, smin=1e-12, cfl=0.9, truth_state="fp32", g
         g.set_problem(pc)
         del pc
         g.set_learned(learned, **opts) if learned is not None else g.set_learned(None)
-        g.run(dt, nsteps, 0)
+        _run_reporting(g, dt, nsteps, progress, f"coarse {name}")
         cc, ac = g.get_state()
         g.release()
         lc = column_load(cc, nx // 2, ny // 2, nz, dz)
