@@ -266,3 +266,24 @@ def test_check_state_counts_bad_cells():
     assert h["nonfinite"] == int((~fin).sum()) == 2
     assert h["negative"] == int((inner[fin] < 0).sum()) == 2
     assert h["min"] == float(inner[fin].min())
+
+
+def test_runs_in_reported_chunks_equal_one_run():
+    """The evaluation's long runs go in chunks of an even number of steps
+    with a progress line after each; the x/y order follows the iteration
+    count, so the state equals that of one run bit for bit."""
+    from fall3d_amd.problem import config1_wind_problem
+
+    p = config1_wind_problem(48, 40, 8, 2, seed=3, dx=5000.0, dz=1000.0, nsources=1, puff=True)
+    one, chunked = (ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32") for _ in range(2))
+    for g in (one, chunked):
+        g.set_problem(p)
+    dt = one.time_step(0.0, 2, 0.9)
+    one.run(dt, 30, 0)
+    msgs = []
+    learned._run_reporting(chunked, dt, 30, msgs.append, "fine", chunk=8)
+    assert msgs == ["fine: 8/30 steps", "fine: 16/30 steps", "fine: 24/30 steps", "fine: 30/30 steps"]
+    (c1, a1), (c2, a2) = one.get_state(), chunked.get_state()
+    assert np.array_equal(c1, c2) and np.array_equal(a1, a2)
+    one.release()
+    chunked.release()
