@@ -65,26 +65,32 @@ def build_hip(jobs: int | None = None, verbose: bool = False) -> str:
 
 def build_fortran(verbose: bool = False) -> list[str]:
     """Compile the MODULE ADS shim (mod_ADS_hip.F90) against the test-harness
-    KindType/Domain modules and link the drop-in driver program."""
+    KindType/Domain modules and link the drop-in driver program; compile the
+    resident-group interfaces (fall3d_ads_c.F90) and link the resident
+    driver program."""
     if not os.path.isdir(FSRC):
         return []
     out = os.path.join(BUILD, "fortran")
     os.makedirs(out, exist_ok=True)
-    order = ["harness_kinds.F90", "mod_ADS_hip.F90", "dropin_driver.F90"]
-    objs = []
-    for f in order:
-        src = os.path.join(FSRC, f)
-        if not os.path.exists(src):
-            return []
-        obj = os.path.join(out, f.replace(".F90", ".o"))
-        cmd = [FC, "-O2", "-cpp", "-fPIC", "-module-dir", out, "-I", out, "-c", src, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
-        objs.append(obj)
-    exe = os.path.join(LIBDIR, "fall3d_dropin_driver")
-    _run([FC, "-o", exe, *objs, "-L", LIBDIR, "-lfall3d_ads_hip", f"-Wl,-rpath,$ORIGIN"])
-    return [exe]
+    programs = {"fall3d_dropin_driver": ["harness_kinds.F90", "mod_ADS_hip.F90", "dropin_driver.F90"],
+                "fall3d_resident_driver": ["fall3d_ads_c.F90", "resident_driver.F90"]}
+    exes = []
+    for name, order in programs.items():
+        objs = []
+        for f in order:
+            src = os.path.join(FSRC, f)
+            if not os.path.exists(src):
+                return exes
+            obj = os.path.join(out, f.replace(".F90", ".o"))
+            cmd = [FC, "-O2", "-cpp", "-fPIC", "-module-dir", out, "-I", out, "-c", src, "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            _run(cmd)
+            objs.append(obj)
+        exe = os.path.join(LIBDIR, name)
+        _run([FC, "-o", exe, *objs, "-L", LIBDIR, "-lfall3d_ads_hip", f"-Wl,-rpath,$ORIGIN"])
+        exes.append(exe)
+    return exes
 
 
 def build(verbose: bool = False) -> None:
