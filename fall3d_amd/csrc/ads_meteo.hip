@@ -380,7 +380,10 @@ __device__ double drydep_velocity(const MeteoArgs &a, const Corner &C, int i, in
   const double re = ust * (z2 - z1) * 0.5 * rho / visc;
   double vd1 = -0.5 * ((re - 40300.0) / 15330.0) * ((re - 40300.0) / 15330.0);
   vd1 = 0.0226 * ust * exp(vd1);
-  const int land = land_class((int)a.lusec[C.at2(i, j)]);  // INT(): the land use of corner (i, j)
+  // INT() of the land use of corner (i, j); a value outside 0..99 (NaN
+  // included) is no USGS code and takes the default class, as unknown codes do
+  const double lu = a.lusec[C.at2(i, j)];
+  const int land = land_class(lu >= 0.0 && lu < 100.0 ? (int)lu : 0);
   double vd = vd1 + kDryA[land - 1][mode - 1] * pow(ust, kDryB[land - 1][mode - 1]);
   vd = 1.0 / (ra + 1.0 / vd);
   return vd;
