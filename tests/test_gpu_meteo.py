@@ -128,4 +128,8 @@ def test_update_meteo_rejects_bad_options():
             a.update_meteo(corner, mg, dict(opts, **bad))
     with pytest.raises(ads.Fall3dAdsError):
         a.get_meteo()  # no meteo yet
+    no_lu = {k: v for k, v in corner.items() if k != "lusec"}
+    with pytest.raises(ads.Fall3dAdsError, match="dry deposition needs"):
+        a.update_meteo(no_lu, mg, dict(opts, dry_deposition=1))
+    a.update_meteo(no_lu, mg, opts)  # without dry deposition the land use is not read
     a.release()

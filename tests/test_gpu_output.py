@@ -89,3 +89,15 @@ def test_cuts_match_restatement(state):
     none = a.postprocess_cuts(zc, sigma, X3max, x_cut=[0.5])  # the other kinds may be left out
     assert none["cut_x"].shape == (ny + 1, nz + 1, 1) and none["cut_z"].size == 0
     a.release()
+
+
+def test_cuts_reject_missing_inputs():
+    p = operational_problem(16, 12, 6, 2, seed=1)
+    a = ads.ADS(p.nx, p.ny, p.nz, p.nb, "minmod", "rk", 8, "fp32")
+    a.set_problem(p)
+    zc = _zc(p.nx, p.ny, p.nz, 2)
+    with pytest.raises(ads.Fall3dAdsError, match="z_c"):
+        a.postprocess_cuts(None, np.linspace(0.0, 1.0, p.nz + 1), 5000.0, z_cut=[100.0])
+    with pytest.raises(ValueError):
+        a.postprocess_cuts(zc, np.linspace(0.0, 1.0, p.nz), 5000.0, z_cut=[100.0])  # nz sigma levels: one short
+    a.release()
