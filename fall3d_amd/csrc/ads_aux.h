@@ -39,8 +39,9 @@ struct SourceMember {
 
 struct CflArgs {
   int nx, ny, nz, nb, criterion;
+  int fp64;  // 1: the fields below are FP64 (the FP64 state), else FP32
   double stime, cfl;
-  const float *u1, *u2, *v1, *v2, *w1, *w2, *k1, *k2, *k3, *vs;
+  const void *u1, *u2, *v1, *v2, *w1, *w2, *k1, *k2, *k3, *vs;
   const double *dx1p, *dx2p, *dx3p;  // FP64 dX*_p, index p+1 for p = -1..n+2
 };
 

@@ -423,8 +423,15 @@ __device__ __forceinline__ double wmax(double v) {
   return v;
 }
 
+// per-cell rates of grid_get_time_step (Sources/mod_Grid.F90:1026-1141) in
+// FP64, summed in the reference's order (criterion 2: its six terms left to
+// right); T is the field type of the state (float, or double for the FP64 state)
+template <typename T>
 __global__ void k_cfl(CflArgs a, double *__restrict__ part) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const T *u1 = (const T *)a.u1, *u2 = (const T *)a.u2, *v1 = (const T *)a.v1, *v2 = (const T *)a.v2;
+  const T *w1 = (const T *)a.w1, *w2 = (const T *)a.w2, *k1p = (const T *)a.k1, *k2p = (const T *)a.k2;
+  const T *k3p = (const T *)a.k3, *vsp = (const T *)a.vs;
   // one wave per (j, k) row at a time, lanes along x
   const int nrows = ny * nz;
   const int nw = gridDim.x * (blockDim.x >> 6);
@@ -438,26 +445,26 @@ __global__ void k_cfl(CflArgs a, double *__restrict__ part) {
       const size_t iu = (size_t)i + (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * (k - 1));
       const size_t iv = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * (k - 1));
       const size_t iw = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * k);
-      const double u = s1 * (double)a.u1[iu] + s * (double)a.u2[iu];
-      const double v = s1 * (double)a.v1[iv] + s * (double)a.v2[iv];
-      const double w = s1 * (double)a.w1[iw] + s * (double)a.w2[iw];
-      const double k1 = a.k1[(size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j - 1) + (size_t)ny * (k - 1))];
-      const double k2 = a.k2[(size_t)(i - 1) + (size_t)nx * ((size_t)(j + 1) + (size_t)(ny + 4) * (k - 1))];
-      const double k3 = a.k3[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k + 1))];
+      const double u = s1 * (double)u1[iu] + s * (double)u2[iu];
+      const double v = s1 * (double)v1[iv] + s * (double)v2[iv];
+      const double w = s1 * (double)w1[iw] + s * (double)w2[iw];
+      const double k1 = k1p[(size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j - 1) + (size_t)ny * (k - 1))];
+      const double k2 = k2p[(size_t)(i - 1) + (size_t)nx * ((size_t)(j + 1) + (size_t)(ny + 4) * (k - 1))];
+      const double k3 = k3p[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k + 1))];
       double wm = 0.0;
       for (int b = 0; b < a.nb; ++b) {
-        const double vs = a.vs[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)k + (size_t)(nz + 3) * b))];
+        const double vs = vsp[(size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * ((size_t)k + (size_t)(nz + 3) * b))];
         wm = fmax(wm, fabs(w - vs));
       }
-      const double t1 = 2.0 * k1 / (dx * dx) + fabs(u) / dx;
-      const double t2 = 2.0 * k2 / (dy * dy) + fabs(v) / dy;
-      const double t3 = 2.0 * k3 / (dz * dz) + wm / dz;
+      const double a1 = 2.0 * k1 / (dx * dx), b1 = fabs(u) / dx;
+      const double a2 = 2.0 * k2 / (dy * dy), b2 = fabs(v) / dy;
+      const double a3 = 2.0 * k3 / (dz * dz), b3 = wm / dz;
       if (a.criterion == 1) {
-        m1 = fmax(m1, t1);
-        m2 = fmax(m2, t2);
-        m3 = fmax(m3, t3);
+        m1 = fmax(m1, a1 + b1);
+        m2 = fmax(m2, a2 + b2);
+        m3 = fmax(m3, a3 + b3);
       } else {
-        m1 = fmax(m1, t1 + t2 + t3);
+        m1 = fmax(m1, a1 + b1 + a2 + b2 + a3 + b3);
       }
     }
   }
@@ -500,7 +507,10 @@ hipError_t cfl_time_step(const CflArgs &a, double *part, int part_cap, double *o
   int nb = (int)((rows + 3) / 4);
   if (nb > part_cap) nb = part_cap;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, a, part);
+  if (a.fp64)
+    hipLaunchKernelGGL(k_cfl<double>, dim3(nb), dim3(256), 0, st, a, part);
+  else
+    hipLaunchKernelGGL(k_cfl<float>, dim3(nb), dim3(256), 0, st, a, part);
   hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(256), 0, st, part, nb, a.cfl, out);
   return hipGetLastError();
 }

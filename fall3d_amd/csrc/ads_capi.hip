@@ -1111,8 +1111,14 @@ int fall3d_ads_get_time_step(fall3d_ads_ctx *c, double stime, int criterion, dou
   a.criterion = criterion;
   a.stime = stime;
   a.cfl = cfl;
-  a.u1 = c->u1; a.u2 = c->u2; a.v1 = c->v1; a.v2 = c->v2; a.w1 = c->w1; a.w2 = c->w2;
-  a.k1 = c->k1; a.k2 = c->k2; a.k3 = c->k3; a.vs = c->vs;
+  if (is_fp64(c)) {  // the FP64 state's own fields: the reference's dt in FP64
+    a.fp64 = 1;
+    a.u1 = c->u1d; a.u2 = c->u2d; a.v1 = c->v1d; a.v2 = c->v2d; a.w1 = c->w1d; a.w2 = c->w2d;
+    a.k1 = c->k1d; a.k2 = c->k2d; a.k3 = c->k3d; a.vs = c->vsd;
+  } else {
+    a.u1 = c->u1; a.u2 = c->u2; a.v1 = c->v1; a.v2 = c->v2; a.w1 = c->w1; a.w2 = c->w2;
+    a.k1 = c->k1; a.k2 = c->k2; a.k3 = c->k3; a.vs = c->vs;
+  }
   a.dx1p = c->dxp64[0];
   a.dx2p = c->dxp64[1];
   a.dx3p = c->dxp64[2];

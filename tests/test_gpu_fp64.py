@@ -80,6 +80,27 @@ def test_dropin_x_sweep_more_planes_than_one_transpose_launch():
     g.release()
 
 
+@pytest.mark.parametrize("grid", ["uniform", "stretched"])
+def test_time_step_bitwise(grid):
+    """grid_get_time_step from the FP64 state's own fields equals the FP64
+    oracle bit for bit: the rates in the reference's order (criterion 2 adds
+    its six terms left to right), the max reductions are exact."""
+    print(f"case fp64 time step {grid}", flush=True)
+    p = _problem(5, grid, 23, 19, 11, 3)
+    o = Oracle(p, 1, 2)
+    g = ads.ADS(p.nx, p.ny, p.nz, p.nb, 1, 2, 8, "fp64", grid=p.grid)
+    g.set_problem(p)
+    for stime in (0.0, 0.3):
+        u = np.asfortranarray((1.0 - stime) * p.u1 + stime * p.u2)
+        v = np.asfortranarray((1.0 - stime) * p.v1 + stime * p.v2)
+        w = np.asfortranarray((1.0 - stime) * p.w1 + stime * p.w2)
+        for crit, cfl in ((1, 0.5), (2, 0.9)):
+            got = g.time_step(stime=stime, criterion=crit, cfl=cfl)
+            ref = o.time_step(u, v, w, p.vs, criterion=crit, cfl=cfl)
+            assert got == ref, (stime, crit, got, ref)
+    g.release()
+
+
 def test_dropin_real4_host_arrays():
     """rp = 4 host arrays into the FP64 state: the inputs widen exactly, so
     the sweep is the FP64 sweep of the widened field."""
