@@ -58,14 +58,17 @@ struct Solve1DArgs {
   int n, limiter;
 };
 
-hipError_t state_in(const void *src, int kind, float *hi, float *lo, size_t n, hipStream_t st);
-hipError_t state_out(const float *hi, const float *lo, void *dst, int kind, size_t n, hipStream_t st);
+// dense host rows of nx4 cells <-> FP32 state rows of pitch floats (rows = (ny+4)(nz+4)nb)
+hipError_t state_in(const void *src, int kind, float *hi, float *lo, int nx4, int pitch, size_t rows, hipStream_t st);
+hipError_t state_out(const float *hi, const float *lo, void *dst, int kind, int nx4, int pitch, size_t rows,
+                     hipStream_t st);
 hipError_t to_f32(const void *src, int kind, float *dst, size_t n, hipStream_t st);
 hipError_t to_f64(const void *src, int kind, double *dst, size_t n, hipStream_t st);
 hipError_t transpose_to_f32(const void *src, int kind, float *dst, int A, int B, int K, hipStream_t st);
 hipError_t transpose_to_f64(const void *src, int kind, double *dst, int A, int B, int K, hipStream_t st);
 // FP64 device array -> n values of the host kind (4 or 8 bytes) in device memory
 hipError_t from_f64(const double *src, void *dst, int kind, size_t n, hipStream_t st);
+hipError_t from_f32(const float *src, void *dst, int kind, size_t n, hipStream_t st);
 hipError_t lerp(const float *a, const float *b, float *out, float s, size_t n, hipStream_t st);
 hipError_t source_inject(float *chi, float *clo, const size_t *idx, const double *rate_vol, const double *meas,
                          int nent, double dt, double *partials, int *nblocks, hipStream_t st);

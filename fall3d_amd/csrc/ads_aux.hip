@@ -12,24 +12,36 @@ namespace f3d {
 // conversion / transposes
 // ---------------------------------------------------------------------------
 
-// host-kind state -> (hi, lo) FP32 pair; lo carries the rounding residual of
-// an FP64 input so the compensated state starts from ~48 significant bits.
+// host-kind state (dense rows of nx4 = nx+4 cells) -> (hi, lo) FP32 pair in
+// rows of pitch floats (the pad words are left as they are); lo carries the
+// rounding residual of an FP64 input so the compensated state starts from ~48
+// significant bits. Workgroups stride over the rows, lanes along x.
 template <typename T>
-__global__ void k_state_in(const T *__restrict__ src, float *__restrict__ hi, float *__restrict__ lo, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const double x = (double)src[i];
-    const float h = (float)x;
-    hi[i] = h;
-    if (lo) lo[i] = (float)(x - (double)h);
+__global__ void k_state_in(const T *__restrict__ src, float *__restrict__ hi, float *__restrict__ lo, int nx4,
+                           int pitch, size_t rows) {
+  for (size_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    const T *s = src + r * (size_t)nx4;
+    const size_t o = r * (size_t)pitch;
+    for (int i = threadIdx.x; i < nx4; i += blockDim.x) {
+      const double x = (double)s[i];
+      const float h = (float)x;
+      hi[o + i] = h;
+      if (lo) lo[o + i] = (float)(x - (double)h);
+    }
   }
 }
 
 template <typename T>
-__global__ void k_state_out(const float *__restrict__ hi, const float *__restrict__ lo, T *__restrict__ dst, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    double x = (double)hi[i];
-    if (lo) x += (double)lo[i];
-    dst[i] = (T)x;
+__global__ void k_state_out(const float *__restrict__ hi, const float *__restrict__ lo, T *__restrict__ dst, int nx4,
+                            int pitch, size_t rows) {
+  for (size_t r = blockIdx.x; r < rows; r += gridDim.x) {
+    T *d = dst + r * (size_t)nx4;
+    const size_t o = r * (size_t)pitch;
+    for (int i = threadIdx.x; i < nx4; i += blockDim.x) {
+      double x = (double)hi[o + i];
+      if (lo) x += (double)lo[o + i];
+      d[i] = (T)x;
+    }
   }
 }
 
@@ -75,21 +87,28 @@ static inline unsigned grid1d(size_t n, int bs = 256) {
   return (unsigned)(g > 65536 ? 65536 : (g == 0 ? 1 : g));
 }
 
-hipError_t state_in(const void *src, int kind, float *hi, float *lo, size_t n, hipStream_t st) {
+static inline unsigned grid_rows(size_t rows) { return (unsigned)(rows > (1u << 20) ? (1u << 20) : (rows ? rows : 1)); }
+
+hipError_t state_in(const void *src, int kind, float *hi, float *lo, int nx4, int pitch, size_t rows, hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
   if (kind == 8)
-    hipLaunchKernelGGL(k_state_in<double>, dim3(grid1d(n)), dim3(256), 0, st, (const double *)src, hi, lo, n);
+    hipLaunchKernelGGL(k_state_in<double>, dim3(grid_rows(rows)), dim3(256), 0, st, (const double *)src, hi, lo, nx4,
+                       pitch, rows);
   else
-    hipLaunchKernelGGL(k_state_in<float>, dim3(grid1d(n)), dim3(256), 0, st, (const float *)src, hi, lo, n);
+    hipLaunchKernelGGL(k_state_in<float>, dim3(grid_rows(rows)), dim3(256), 0, st, (const float *)src, hi, lo, nx4,
+                       pitch, rows);
   return hipGetLastError();
 }
 
-hipError_t state_out(const float *hi, const float *lo, void *dst, int kind, size_t n, hipStream_t st) {
+hipError_t state_out(const float *hi, const float *lo, void *dst, int kind, int nx4, int pitch, size_t rows,
+                     hipStream_t st) {
   (void)hipGetLastError();  // drop errors of earlier, unrelated calls
   if (kind == 8)
-    hipLaunchKernelGGL(k_state_out<double>, dim3(grid1d(n)), dim3(256), 0, st, hi, lo, (double *)dst, n);
+    hipLaunchKernelGGL(k_state_out<double>, dim3(grid_rows(rows)), dim3(256), 0, st, hi, lo, (double *)dst, nx4,
+                       pitch, rows);
   else
-    hipLaunchKernelGGL(k_state_out<float>, dim3(grid1d(n)), dim3(256), 0, st, hi, lo, (float *)dst, n);
+    hipLaunchKernelGGL(k_state_out<float>, dim3(grid_rows(rows)), dim3(256), 0, st, hi, lo, (float *)dst, nx4,
+                       pitch, rows);
   return hipGetLastError();
 }
 
@@ -109,6 +128,16 @@ hipError_t to_f64(const void *src, int kind, double *dst, size_t n, hipStream_t 
   else
     hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1d(n)), dim3(256), 0, st, (const float *)src, dst, n);
   return hipGetLastError();
+}
+
+// FP32 device array -> host kind (FP64 or a plain FP32 copy)
+hipError_t from_f32(const float *src, void *dst, int kind, size_t n, hipStream_t st) {
+  (void)hipGetLastError();  // drop errors of earlier, unrelated calls
+  if (kind == 8) {
+    hipLaunchKernelGGL(k_to_f64<float>, dim3(grid1d(n)), dim3(256), 0, st, src, (double *)dst, n);
+    return hipGetLastError();
+  }
+  return hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, st);
 }
 
 hipError_t transpose_to_f32(const void *src, int kind, float *dst, int A, int B, int K, hipStream_t st) {
@@ -526,13 +555,14 @@ struct RowWalk {
   int j, k, b;    // 1-based j, k; bin
   size_t base;    // element offset of cell (0, j, k) of bin b
 };
-__device__ __forceinline__ RowWalk row_of(int r, int nx, int ny, int nz) {
+// row r of the state with x pitch px (nx+4 for c64, state_pitch(nx) for chi/clo)
+__device__ __forceinline__ RowWalk row_of(int r, int px, int ny, int nz) {
   RowWalk w;
   const int t = r / ny;
   w.j = r - t * ny + 1;
   w.k = t % nz + 1;
   w.b = t / nz;
-  w.base = (size_t)1 + (size_t)(nx + 4) * ((size_t)(w.j + 1) + (size_t)(ny + 4) * ((size_t)(w.k + 1) + (size_t)(nz + 4) * w.b));
+  w.base = (size_t)1 + (size_t)px * ((size_t)(w.j + 1) + (size_t)(ny + 4) * ((size_t)(w.k + 1) + (size_t)(nz + 4) * w.b));
   return w;
 }
 // state value of element ci: MODE 0 FP32 (chi), 1 compensated (chi + clo), 2 FP64 (c64)
@@ -557,7 +587,7 @@ __global__ void __launch_bounds__(256) k_mass(const float *__restrict__ chi, con
   const int nw = gridDim.x * (blockDim.x >> 6);
   double mv = 0.0, mc = 0.0;
   for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < nrows; r += nw) {
-    const RowWalk w = row_of(r, nx, ny, nz);
+    const RowWalk w = row_of(r, MODE == 2 ? nx + 4 : state_pitch(nx), ny, nz);
     const double pj = sp.p2[w.j + 1], pk = sp.p3[w.k + 1];
     const double bj = sp.b2[w.j], bk = sp.b3[w.k];
 #pragma unroll 4
@@ -770,7 +800,7 @@ __global__ void __launch_bounds__(256) k_health(const float *__restrict__ chi, c
   const int nw = gridDim.x * (blockDim.x >> 6);
   double nonfin = 0.0, neg = 0.0, mn = INFINITY;
   for (int r = blockIdx.x * (blockDim.x >> 6) + wid; r < nrows; r += nw) {
-    const RowWalk w = row_of(r, nx, ny, nz);
+    const RowWalk w = row_of(r, MODE == 2 ? nx + 4 : state_pitch(nx), ny, nz);
 #pragma unroll 4
     for (int i = lane + 1; i <= nx; i += 64) {
       const double v = state_at<MODE>(chi, clo, c64, w.base + (size_t)i);

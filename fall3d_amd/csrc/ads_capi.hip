@@ -466,7 +466,8 @@ int up_state(fall3d_ads_ctx *c, const void *h) {
   if (is_fp64(c))
     HIPCHK(c, to_f64(s, c->kind, c->c64, c->ncell_c, c->stream));
   else
-    HIPCHK(c, state_in(s, c->kind, c->chi, c->comp ? c->clo : nullptr, c->ncell_c, c->stream));
+    HIPCHK(c, state_in(s, c->kind, c->chi, c->comp ? c->clo : nullptr, c->nx + 4, state_pitch(c->nx),
+                       (size_t)(c->ny + 4) * (c->nz + 4) * c->nb, c->stream));
   return FALL3D_ADS_OK;
 }
 
@@ -477,7 +478,8 @@ int down_state(fall3d_ads_ctx *c, void *h) {
   if (is_fp64(c))
     HIPCHK(c, from_f64(c->c64, c->dstage, c->kind, c->ncell_c, c->stream));
   else
-    HIPCHK(c, state_out(c->chi, c->comp ? c->clo : nullptr, c->dstage, c->kind, c->ncell_c, c->stream));
+    HIPCHK(c, state_out(c->chi, c->comp ? c->clo : nullptr, c->dstage, c->kind, c->nx + 4, state_pitch(c->nx),
+                        (size_t)(c->ny + 4) * (c->nz + 4) * c->nb, c->stream));
   HIPCHK(c, hipMemcpyAsync(h, c->dstage, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return FALL3D_ADS_OK;
@@ -696,7 +698,7 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     // count is a 32-bit byte size and drop masked stores with an offset of
     // 2^31: every such array must stay below 2^31 bytes.
     const size_t lim = (size_t)1 << 31;
-    const size_t per_bin_c = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * sizeof(float);
+    const size_t per_bin_c = (size_t)state_pitch(nx) * (ny + 4) * (nz + 4) * sizeof(float);
     const size_t faces = std::max({(size_t)(nx + 4) * ny * nz, (size_t)nx * (ny + 4) * nz, (size_t)nx * ny * (nz + 4)}) *
                          sizeof(float);
     if (per_bin_c >= lim || faces >= lim)
@@ -725,6 +727,7 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->own_stream = true;
   c->ncell_c = (size_t)(nx + 4) * (ny + 4) * (nz + 4) * nbins;
+  c->ncell_s = (size_t)state_pitch(nx) * (ny + 4) * (nz + 4) * nbins;
   c->nacum = (size_t)(nx + 4) * (ny + 4) * nbins;
   if (is_fp64(c)) {
     HIPCHK(c, dalloc(&c->c64, c->ncell_c));
@@ -737,14 +740,14 @@ int fall3d_ads_initialize(fall3d_ads_ctx **pctx, int nx, int ny, int nz, int nbi
     HIPCHK(c, dalloc(&c->u2t, n_u(c)));
     HIPCHK(c, dalloc(&c->k1t, n_k1(c)));
   } else {
-    HIPCHK(c, dalloc(&c->chi_base, c->ncell_c + 2));
-    c->chi = c->chi_base + 2;
-    HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
+    HIPCHK(c, dalloc(&c->chi_base, c->ncell_s + kStateLead + 2));
+    c->chi = c->chi_base + kStateLead;
+    HIPCHK(c, hipMemsetAsync(c->chi_base, 0, (c->ncell_s + kStateLead + 2) * sizeof(float), c->stream));
   }
   if (c->comp == FALL3D_ADS_STATE_FP32X2) {
-    HIPCHK(c, dalloc(&c->clo_base, c->ncell_c + 2));
-    c->clo = c->clo_base + 2;
-    HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_c * sizeof(float), c->stream));
+    HIPCHK(c, dalloc(&c->clo_base, c->ncell_s + kStateLead + 2));
+    c->clo = c->clo_base + kStateLead;
+    HIPCHK(c, hipMemsetAsync(c->clo_base, 0, (c->ncell_s + kStateLead + 2) * sizeof(float), c->stream));
   }
   HIPCHK(c, dalloc(&c->acum, c->nacum));
   HIPCHK(c, hipMemsetAsync(c->acum, 0, c->nacum * sizeof(double), c->stream));
@@ -881,7 +884,7 @@ int fall3d_ads_update_horizontal_vcomps(fall3d_ads_ctx *c, const void *u1, const
     HIPCHK(c, to_f32(s, c->kind, b, cnt[q], c->stream));
     HIPCHK(c, lerp(a, b, a, (float)stime, cnt[q], c->stream));
     if ((rc = ensure_stage(c, cnt[q] * 8))) { dfree(a); dfree(b); return rc; }
-    HIPCHK(c, state_out(a, nullptr, c->dstage, c->kind, cnt[q], c->stream));
+    HIPCHK(c, from_f32(a, c->dstage, c->kind, cnt[q], c->stream));
     HIPCHK(c, hipMemcpyAsync(dst[q], c->dstage, cnt[q] * c->kind, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1025,8 +1028,9 @@ int fall3d_ads_set_source_metric(fall3d_ads_ctx *c, const void *s, const void *H
     for (int i = 1; i <= nx; ++i)
       if (!(hm1(j) * hm2(j) * hm3(i, j) > 0.0))
         return fail(c, FALL3D_ADS_EINVAL, "non-positive map factor product at cell (%d,%d)", i, j);
-  std::vector<size_t> idx;
+  std::vector<size_t> idx;  // element offsets in the state (c64 or chi/clo)
   std::vector<double> rate, meas, sval, volv;
+  const int px = is_fp64(c) ? nx + 4 : state_pitch(nx);
   // Sources/mod_F3D.F90:1456-1473: c += dt*s/vol, vol = dX1_p dX2_p dX3_p Hm1 Hm2 Hm3
   for (int b = 0; b < c->nb; ++b)
     for (int k = 1; k <= nz; ++k)
@@ -1037,7 +1041,7 @@ int fall3d_ads_set_source_metric(fall3d_ads_ctx *c, const void *s, const void *H
           if (v == 0.0) continue;
           const double vol = c->hdxp64[0][i + 1] * c->hdxp64[1][j + 1] * c->hdxp64[2][k + 1] *
                              (hm1(j) * hm2(j) * hm3(i, j));
-          idx.push_back((size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b)));
+          idx.push_back((size_t)(i + 1) + (size_t)px * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b)));
           rate.push_back(v / vol);
           sval.push_back(v);
           volv.push_back(vol);
@@ -1076,8 +1080,8 @@ int fall3d_ads_set_state(fall3d_ads_ctx *c, const void *cst, const void *acum) {
   } else if (is_fp64(c)) {  // zero field, no host transfer
     HIPCHK(c, hipMemsetAsync(c->c64, 0, c->ncell_c * sizeof(double), c->stream));
   } else {
-    HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_c * sizeof(float), c->stream));
-    if (c->comp) HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_c * sizeof(float), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->chi, 0, c->ncell_s * sizeof(float), c->stream));
+    if (c->comp) HIPCHK(c, hipMemsetAsync(c->clo, 0, c->ncell_s * sizeof(float), c->stream));
   }
   if (acum) {
     if ((rc = up_acum(c, acum))) return rc;
@@ -1306,7 +1310,7 @@ constexpr char kSnapMagic[8] = {'F', '3', 'D', 'A', 'D', 'S', 'R', '1'};
 
 // bytes of the state words: FP32 hi (+ lo), or FP64
 size_t state_bytes(const fall3d_ads_ctx *c) {
-  return is_fp64(c) ? sizeof(double) * c->ncell_c : sizeof(float) * c->ncell_c * (c->comp ? 2 : 1);
+  return is_fp64(c) ? sizeof(double) * c->ncell_c : sizeof(float) * c->ncell_s * (c->comp ? 2 : 1);
 }
 
 size_t snapshot_bytes(const fall3d_ads_ctx *c) {
@@ -1346,9 +1350,9 @@ int fall3d_ads_snapshot(fall3d_ads_ctx *c, void *buf, size_t *bytes, double time
   if (is_fp64(c)) {
     HIPCHK(c, hipMemcpyAsync(p, c->c64, state_bytes(c), hipMemcpyDeviceToHost, c->stream));
   } else {
-    HIPCHK(c, hipMemcpyAsync(p, c->chi, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
-    p += sizeof(float) * c->ncell_c;
-    if (c->comp) HIPCHK(c, hipMemcpyAsync(p, c->clo, sizeof(float) * c->ncell_c, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(p, c->chi, sizeof(float) * c->ncell_s, hipMemcpyDeviceToHost, c->stream));
+    p += sizeof(float) * c->ncell_s;
+    if (c->comp) HIPCHK(c, hipMemcpyAsync(p, c->clo, sizeof(float) * c->ncell_s, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *bytes = need;
@@ -1377,9 +1381,9 @@ int fall3d_ads_restore(fall3d_ads_ctx *c, const void *buf, size_t bytes, double 
   if (is_fp64(c)) {
     HIPCHK(c, hipMemcpyAsync(c->c64, p, state_bytes(c), hipMemcpyHostToDevice, c->stream));
   } else {
-    HIPCHK(c, hipMemcpyAsync(c->chi, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
-    p += sizeof(float) * c->ncell_c;
-    if (c->comp) HIPCHK(c, hipMemcpyAsync(c->clo, p, sizeof(float) * c->ncell_c, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->chi, p, sizeof(float) * c->ncell_s, hipMemcpyHostToDevice, c->stream));
+    p += sizeof(float) * c->ncell_s;
+    if (c->comp) HIPCHK(c, hipMemcpyAsync(c->clo, p, sizeof(float) * c->ncell_s, hipMemcpyHostToDevice, c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (time) *time = h.time;
@@ -1436,7 +1440,7 @@ int fall3d_ads_autotune(fall3d_ads_ctx *c, double dt, double stime, int reps, in
   if ((rc = ensure_dt(c, dt))) return rc;
   // save
   void *src[4] = {c->chi, c->comp ? c->clo : nullptr, c->acum, c->acc};
-  const size_t bytes[4] = {c->ncell_c * sizeof(float), c->ncell_c * sizeof(float), c->nacum * sizeof(double),
+  const size_t bytes[4] = {c->ncell_s * sizeof(float), c->ncell_s * sizeof(float), c->nacum * sizeof(double),
                            (size_t)ACC_LEN * sizeof(double)};
   void *save[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int i = 0; i < 4; ++i) {

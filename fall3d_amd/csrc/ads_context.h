@@ -53,9 +53,9 @@ struct fall3d_ads_ctx {
   std::string err = "no error";
 
   // state
-  // c (and its low words) start 2 floats into their allocations: with the 2
-  // ghost columns, cell 1 of a row then sits on a 16-byte boundary whenever
-  // nx is a multiple of 4 (the sweeps read and write rows in 16-byte pieces)
+  // c (and its low words) start kStateLead floats into their allocations,
+  // with rows of state_pitch(nx) floats: cell 1 of every row starts a
+  // 128-byte line (the sweeps read and write rows in 16-byte pieces)
   float *chi = nullptr, *clo = nullptr;
   float *chi_base = nullptr, *clo_base = nullptr;
   // FP64 state (FALL3D_ADS_STATE_FP64): c64 holds the concentration in place
@@ -70,7 +70,8 @@ struct fall3d_ads_ctx {
   // (ny,0:nx+2,nz), k1t (ny,-1:nx+2,nz)
   double *c64t = nullptr, *u1t = nullptr, *u2t = nullptr, *k1t = nullptr;
   double *acum = nullptr;
-  size_t ncell_c = 0;  // (nx+4)(ny+4)(nz+4)nb
+  size_t ncell_c = 0;  // (nx+4)(ny+4)(nz+4)nb: cells of the state with ghosts (the host layout, c64)
+  size_t ncell_s = 0;  // state_pitch(nx)(ny+4)(nz+4)nb: words of chi and of clo
   size_t nacum = 0;    // (nx+4)(ny+4)nb
 
   // meteo (device layouts, FP32)

@@ -281,7 +281,7 @@ int snapshot(fall3d_ads_ensemble *e, Snapshot &s, bool restore) {
   int k = 0;
   for (auto *c : e->mem) {
     void *src[4] = {c->chi, c->comp ? c->clo : nullptr, c->acum, c->acc};
-    const size_t bytes[4] = {c->ncell_c * sizeof(float), c->ncell_c * sizeof(float), c->nacum * sizeof(double),
+    const size_t bytes[4] = {c->ncell_s * sizeof(float), c->ncell_s * sizeof(float), c->nacum * sizeof(double),
                              (size_t)ACC_LEN * sizeof(double)};
     for (int i = 0; i < 4; ++i, ++k) {
       if (!src[i]) {

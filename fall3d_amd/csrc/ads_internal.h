@@ -17,7 +17,16 @@ namespace f3d {
 //   k2    (nx, -1:ny+2, nz)      idx = (i-1) + nx*((j+1) + (ny+4)*(k-1))
 //   k3    (nx, ny, -1:nz+2)      idx = (i-1) + nx*((j-1) + ny*(k+1))
 //   vs    (nx, ny, 0:nz+2, nb)   idx = (i-1) + nx*((j-1) + ny*(f + (nz+3)*b))
-// The concentration keeps the reference layout c(-1:nx+2,-1:ny+2,-1:nz+2,nb).
+// The concentration keeps the reference's index order c(-1:nx+2,-1:ny+2,-1:nz+2,nb).
+// The FP64 state stores it densely (x pitch nx+4). The FP32 state (and its
+// compensated low part) pads every x row to state_pitch(nx) floats, a multiple
+// of 32, and the state pointer sits 30 floats past a 256-byte-aligned
+// allocation: interior cell 1 of every row starts a 128-byte line, so a
+// wave's 256-byte row piece covers two whole lines.
+//   chi (-1:nx+2, -1:ny+2, -1:nz+2, nb)  idx = (i+1) + P*((j+1) + (ny+4)*((k+1) + (nz+4)*b)),
+//   P = state_pitch(nx); the pad words of each row stay zero.
+__host__ __device__ constexpr int state_pitch(int nx) { return (nx + 4 + 31) & ~31; }
+constexpr int kStateLead = 30;  // floats between the allocation and the state pointer
 
 struct SweepArgs {
   float *chi;             // state (hi part)

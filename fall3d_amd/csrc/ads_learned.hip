@@ -86,7 +86,7 @@ struct FaceArgs {
 };
 
 __device__ __forceinline__ size_t cix(const FaceArgs &a, int i, int j, int k, int b) {
-  return (size_t)(i + 1) + (size_t)(a.nx + 4) * ((size_t)(j + 1) + (size_t)(a.ny + 4) * ((size_t)(k + 1) + (size_t)(a.nz + 4) * b));
+  return (size_t)(i + 1) + (size_t)state_pitch(a.nx) * ((size_t)(j + 1) + (size_t)(a.ny + 4) * ((size_t)(k + 1) + (size_t)(a.nz + 4) * b));
 }
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 __device__ __forceinline__ float lerpv(const FaceArgs &a, const float *p1, const float *p2, size_t i) {
@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(256) k_apply_corrections(
     const size_t fy = (size_t)(i - 1) + (size_t)nx * ((size_t)j + (size_t)(ny + 3) * kbs);
     const float d = (limited(dx, px, fx + 1, lim, ox1) - limited(dx, px, fx, lim, ox0)) * idxb_x[i] +
                     (limited(dy, py, fy + nx, lim, oy1) - limited(dy, py, fy, lim, oy0)) * idxb_y[j];
-    const size_t ci = (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
+    const size_t ci = (size_t)(i + 1) + (size_t)state_pitch(nx) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b));
     c[ci] -= d;
   }
 }
@@ -705,9 +705,9 @@ __global__ void k_coarse_grain(const float *__restrict__ fine, float *__restrict
     for (int dj = 0; dj < 2; ++dj)
       for (int di = 0; di < 2; ++di) {
         const int i = 2 * I - 1 + di, j = 2 * J - 1 + dj;
-        s += fine[(size_t)(i + 1) + (size_t)(fnx + 4) * ((size_t)(j + 1) + (size_t)(fny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b))];
+        s += fine[(size_t)(i + 1) + (size_t)state_pitch(fnx) * ((size_t)(j + 1) + (size_t)(fny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b))];
       }
-    coarse[(size_t)(I + 1) + (size_t)(nx + 4) * ((size_t)(J + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b))] = 0.25f * s;
+    coarse[(size_t)(I + 1) + (size_t)state_pitch(nx) * ((size_t)(J + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * b))] = 0.25f * s;
   }
 }
 
@@ -816,7 +816,7 @@ namespace host {
 int learned_pre_step(fall3d_ads_ctx *c, double dt, double stime) {
   const bool rel = c->learned_smin < 0.0;
   if (rel) {  // threshold relative to the field's largest value
-    hipLaunchKernelGGL(k_absmax, dim3(c->lthr_blocks), dim3(256), 0, c->stream, c->chi, c->ncell_c, c->lthr + 1);
+    hipLaunchKernelGGL(k_absmax, dim3(c->lthr_blocks), dim3(256), 0, c->stream, c->chi, c->ncell_s, c->lthr + 1);
     hipLaunchKernelGGL(k_threshold, dim3(1), dim3(64), 0, c->stream, c->lthr + 1, c->lthr_blocks,
                        (float)(-c->learned_smin), c->lthr);
     const hipError_t e = hipGetLastError();

@@ -54,8 +54,9 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // scaled-back, clipped concentration in g/m3 at mass point (i, j, k) of bin b,
 // i in 0..nx+1, j in 0..ny+1, k in 0..nz+1
 __device__ __forceinline__ double cs_at(const OutArgs &a, int i, int j, int k, int b) {
-  const size_t ci = (size_t)(i + 1) + (size_t)(a.nx + 4) * ((size_t)(j + 1) + (size_t)(a.ny + 4) *
-                                                              ((size_t)(k + 1) + (size_t)(a.nz + 4) * b));
+  const int px = a.c64 ? a.nx + 4 : state_pitch(a.nx);  // x pitch of c64 / chi
+  const size_t ci = (size_t)(i + 1) + (size_t)px * ((size_t)(j + 1) + (size_t)(a.ny + 4) *
+                                                    ((size_t)(k + 1) + (size_t)(a.nz + 4) * b));
   double c;
   if (a.c64) {
     c = a.c64[ci];

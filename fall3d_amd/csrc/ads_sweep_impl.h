@@ -41,7 +41,7 @@
 namespace f3d {
 
 __device__ __forceinline__ size_t cidx(int i, int j, int k, int b, int nx, int ny, int nz) {
-  return (size_t)(i + 1) + (size_t)(nx + 4) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * (size_t)b));
+  return (size_t)(i + 1) + (size_t)state_pitch(nx) * ((size_t)(j + 1) + (size_t)(ny + 4) * ((size_t)(k + 1) + (size_t)(nz + 4) * (size_t)b));
 }
 
 // Limiter slope m = phi(d_i/d_{i+1}) * d_{i+1} (a = d_i, b = d_{i+1}).
@@ -258,7 +258,7 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
     bins[j] = bok[j] ? g * a.nbin + loc : 0;
   }
 
-  const size_t cbin = (size_t)(nx + 4) * (ny + 4) * (nz + 4);
+  const size_t cbin = (size_t)state_pitch(nx) * (ny + 4) * (nz + 4);
   const size_t nvel = (DIR == 1) ? (size_t)nx * (ny + 3) * nz : (size_t)nx * ny * (nz + 3);
   const size_t nkd = (DIR == 1) ? (size_t)nx * (ny + 4) * nz : (size_t)nx * ny * (nz + 4);
   const size_t nvs = (size_t)nx * ny * (nz + 3);
@@ -283,15 +283,15 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   // lane byte offsets of position -1 (c), face 0 (winds), cell -1 (K); byte strides
   uint32_t vc, cs4, vv, vst4, vk, kst4;
   if constexpr (DIR == 1) {
-    vc = (uint32_t)(4 * ((size_t)(ia + 1) + (size_t)(nx + 4) * ((size_t)(ny + 4) * (t2 + 1))));
-    cs4 = (uint32_t)(4 * (nx + 4));
+    vc = (uint32_t)(4 * ((size_t)(ia + 1) + (size_t)state_pitch(nx) * ((size_t)(ny + 4) * (t2 + 1))));
+    cs4 = (uint32_t)(4 * state_pitch(nx));
     vv = (uint32_t)(4 * ((size_t)(ia - 1) + (size_t)nx * (ny + 3) * (t2 - 1)));
     vst4 = (uint32_t)(4 * nx);
     vk = (uint32_t)(4 * ((size_t)(ia - 1) + (size_t)nx * (ny + 4) * (t2 - 1)));
     kst4 = (uint32_t)(4 * nx);
   } else {
-    vc = (uint32_t)(4 * ((size_t)(ia + 1) + (size_t)(nx + 4) * (t2 + 1)));
-    cs4 = (uint32_t)(4 * (size_t)(nx + 4) * (ny + 4));
+    vc = (uint32_t)(4 * ((size_t)(ia + 1) + (size_t)state_pitch(nx) * (t2 + 1)));
+    cs4 = (uint32_t)(4 * (size_t)state_pitch(nx) * (ny + 4));
     vv = (uint32_t)(4 * ((size_t)(ia - 1) + (size_t)nx * (t2 - 1)));
     vst4 = (uint32_t)(4 * (size_t)nx * ny);
     vk = vv;
@@ -304,11 +304,11 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   const int c4 = (col4 <= nx) ? col4 : 1;
   uint32_t xc, xv, xk;
   if constexpr (DIR == 1) {
-    xc = (uint32_t)(4 * ((size_t)(c4 + 1) + (size_t)(nx + 4) * ((size_t)(ny + 4) * (t2 + 1))));
+    xc = (uint32_t)(4 * ((size_t)(c4 + 1) + (size_t)state_pitch(nx) * ((size_t)(ny + 4) * (t2 + 1))));
     xv = (uint32_t)(4 * ((size_t)(c4 - 1) + (size_t)nx * (ny + 3) * (t2 - 1)));
     xk = (uint32_t)(4 * ((size_t)(c4 - 1) + (size_t)nx * (ny + 4) * (t2 - 1)));
   } else {
-    xc = (uint32_t)(4 * ((size_t)(c4 + 1) + (size_t)(nx + 4) * (t2 + 1)));
+    xc = (uint32_t)(4 * ((size_t)(c4 + 1) + (size_t)state_pitch(nx) * (t2 + 1)));
     xv = (uint32_t)(4 * ((size_t)(c4 - 1) + (size_t)nx * (t2 - 1)));
     xk = xv;
   }
@@ -809,7 +809,7 @@ __device__ __forceinline__ void sweep_x_body(const SweepArgs &a, const int bx, c
   const int lsrc = (lane + 63) & 63, rsrc = (lane + 1) & 63;
 
   const size_t cbase = cidx(0, j, k, 0, nx, ny, nz);
-  const size_t cbin = (size_t)(nx + 4) * (ny + 4) * (nz + 4);
+  const size_t cbin = (size_t)state_pitch(nx) * (ny + 4) * (nz + 4);
   const size_t vbase = (size_t)(nx + 3) * ((size_t)(j - 1) + (size_t)ny * (k - 1));
   const size_t kbase = (size_t)(nx + 4) * ((size_t)(j - 1) + (size_t)ny * (k - 1));
   const double area = a.areaB[k] * a.areaA[j];  // dX3_b(k) dX2_b(j)

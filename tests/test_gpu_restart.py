@@ -66,3 +66,32 @@ def test_restore_rejects_another_grid():
         c.restore(b"\0" * 64 + blob[64:])
     b.release()
     c.release()
+
+
+@pytest.mark.parametrize("nx,state", [(22, "fp32"), (22, "fp32x2"), (61, "fp32"), (28, "fp32")])
+def test_snapshot_holds_the_padded_state_rows(nx, state):
+    """The FP32 state's x rows are padded on the device to a multiple of 32
+    floats (nx = 22: 26 cells with ghosts in 32 words; 61: 65 in 96; 28: 32,
+    no pad). The snapshot carries those words last: after steps with sources
+    every pad word is still zero, and the cells equal get_state (hi + lo
+    summed in FP64, as get_state forms them) bit for bit."""
+    ny, nz, nb = 9, 6, 2
+    print(f"case: padded rows nx={nx} {state}", flush=True)
+    p = operational_problem(nx, ny, nz, nb, seed=8)
+    a = ads.ADS(nx, ny, nz, nb, "minmod", "rk", 8, state)
+    a.set_problem(p)
+    dt = a.time_step(0.0, 2, 0.9)
+    a.run(dt, 5, 0)
+    c, _ = a.get_state()
+    blob = np.frombuffer(a.snapshot(5 * dt, 5), dtype=np.uint8)
+    a.release()
+    pitch = (nx + 4 + 31) // 32 * 32
+    ns = pitch * (ny + 4) * (nz + 4) * nb
+    parts = 2 if state == "fp32x2" else 1
+    words = blob[blob.size - 4 * ns * parts:].view(np.float32).reshape(parts, nb, nz + 4, ny + 4, pitch)
+    assert np.count_nonzero(words[..., nx + 4:]) == 0
+    got = words[0, ..., :nx + 4].astype(np.float64)
+    if parts == 2:
+        got = got + words[1, ..., :nx + 4].astype(np.float64)
+    assert np.count_nonzero(c) > 0
+    assert np.array_equal(got.reshape(-1), np.asarray(c).reshape(-1, order="F"))
