@@ -111,8 +111,9 @@ def hbm_reference(mib, reps=20):
 
 def copy_probe(mib):
     """Known-traffic references for the HBM counters: each copy reads and
-    writes exactly mib MiB (torch's copy kernel, and the library's copy with
-    16-byte buffer loads and stores, the access width of the sweeps)."""
+    writes exactly mib MiB (torch's copy kernel, the library's copy with
+    16-byte buffer loads and stores, the access width of the FP32 sweeps, and
+    its copy with 8-byte loads and stores, that of the FP64 sweeps)."""
     import ctypes
 
     import torch
@@ -129,6 +130,14 @@ def copy_probe(mib):
     for _ in range(5):
         rc = lib.fall3d_ads_copy_probe(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
                                        ctypes.c_size_t(n), None)
+        assert rc == 0
+    torch.cuda.synchronize()
+    assert bool((dst == src).all())
+    src64 = src.view(torch.float64)  # the same bytes as n/2 doubles
+    dst.zero_()
+    for _ in range(5):
+        rc = lib.fall3d_ads_copy_probe64(ctypes.c_void_p(src64.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                         ctypes.c_size_t(n // 2), None)
         assert rc == 0
     torch.cuda.synchronize()
     assert bool((dst == src).all())

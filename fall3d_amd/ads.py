@@ -49,7 +49,7 @@ EXPORTS = (
     "fall3d_ads_get_time_step", "fall3d_ads_step", "fall3d_ads_step_noalloc", "fall3d_ads_run", "fall3d_ads_get_fluxes",
     "fall3d_ads_set_fluxes", "fall3d_ads_get_budget", "fall3d_ads_set_stream", "fall3d_ads_synchronize",
     "fall3d_ads_set_tiling", "fall3d_ads_set_profiling", "fall3d_ads_get_kernel_time", "fall3d_ads_sweep_bytes",
-    "fall3d_ads_copy_probe", "fall3d_ads_check_state", "fall3d_ads_learned_fallbacks", "fall3d_ads_snapshot",
+    "fall3d_ads_copy_probe", "fall3d_ads_copy_probe64", "fall3d_ads_check_state", "fall3d_ads_learned_fallbacks", "fall3d_ads_snapshot",
     "fall3d_ads_restore", "fall3d_ens_statistics", "fall3d_ads_update_meteo", "fall3d_ads_get_meteo",
     "fall3d_ads_postprocess", "fall3d_ads_postprocess_cuts", "fall3d_ads_set_boundary",
     "fall3d_ads_set_graph", "fall3d_ads_autotune", "fall3d_ads_get_tiling",
@@ -157,6 +157,7 @@ def load_library(path: str = LIB_PATH):
         "fall3d_ads_step": ([_vp, _dbl, _dbl, _int], _int),
         "fall3d_ads_step_noalloc": ([_vp, _dbl, _dbl, _int], _int),
         "fall3d_ads_copy_probe": ([_vp, _vp, ctypes.c_size_t, _vp], _int),
+        "fall3d_ads_copy_probe64": ([_vp, _vp, ctypes.c_size_t, _vp], _int),
         "fall3d_ads_check_state": ([_vp, _vp, _dp], _int),
         "fall3d_ads_snapshot": ([_vp, _vp, ctypes.POINTER(ctypes.c_size_t), _dbl, ctypes.c_longlong], _int),
         "fall3d_ads_restore": ([_vp, _vp, ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_longlong)], _int),

@@ -93,5 +93,6 @@ hipError_t health(const float *chi, const float *clo, const double *c64, int nx,
                   double *part, int part_cap, double *out, hipStream_t st);
 // n floats src -> dst with 16-byte buffer loads/stores (HBM counter calibration)
 hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st);
+hipError_t copy_probe64(const double *src, double *dst, size_t n, hipStream_t st);
 
 }  // namespace f3d

@@ -768,6 +768,20 @@ __global__ void __launch_bounds__(256) k_copy_probe(const float *__restrict__ sr
   }
 }
 
+// 8-byte accesses, the access width of the FP64 state's sweeps and transposes
+// (plain global loads and stores, consecutive lanes on consecutive words)
+__global__ void __launch_bounds__(256) k_copy_probe64(const double *__restrict__ src, double *__restrict__ dst,
+                                                       size_t n) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    dst[q] = src[q];
+}
+
+hipError_t copy_probe64(const double *src, double *dst, size_t n, hipStream_t st) {
+  (void)hipGetLastError();
+  if (n) hipLaunchKernelGGL(k_copy_probe64, dim3(2048), dim3(256), 0, st, src, dst, n);
+  return hipGetLastError();
+}
+
 hipError_t copy_probe(const float *src, float *dst, size_t n, hipStream_t st) {
   (void)hipGetLastError();
   constexpr size_t kChunk = (size_t)1 << 28;  // floats (1 GiB)

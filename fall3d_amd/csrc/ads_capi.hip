@@ -1237,6 +1237,13 @@ int fall3d_ads_copy_probe(const void *src, void *dst, size_t n, void *stream) {
   return hipStreamSynchronize(st) == hipSuccess ? FALL3D_ADS_OK : FALL3D_ADS_EHIP;
 }
 
+int fall3d_ads_copy_probe64(const void *src, void *dst, size_t n, void *stream) {
+  if ((!src || !dst) && n) return FALL3D_ADS_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (copy_probe64((const double *)src, (double *)dst, n, st) != hipSuccess) return FALL3D_ADS_EHIP;
+  return hipStreamSynchronize(st) == hipSuccess ? FALL3D_ADS_OK : FALL3D_ADS_EHIP;
+}
+
 int fall3d_ads_set_boundary(fall3d_ads_ctx *c, int mode) {
   if (!c) return FALL3D_ADS_EINVAL;
   if (mode < FALL3D_ADS_BC_FREEFLOW || mode > FALL3D_ADS_BC_PERIODIC)

@@ -422,6 +422,13 @@ int fall3d_ads_sweep_bytes(fall3d_ads_ctx *ctx, int dir, double *bytes);
  * calibrating the HBM byte counters. */
 int fall3d_ads_copy_probe(const void *src, void *dst, size_t n, void *stream);
 
+/* Measurement aid, no counterpart in the reference: copies n doubles from
+ * src to dst (device pointers) with 8-byte loads and stores, the access
+ * width of the FP64 state's sweeps, on stream (NULL: the null stream), and
+ * synchronizes. The known-traffic kernel (8n bytes read, 8n written) for
+ * calibrating the HBM byte counters at that width. */
+int fall3d_ads_copy_probe64(const void *src, void *dst, size_t n, void *stream);
+
 /* ---- 3. Ensemble group --------------------------------------------------
  * FALL3D runs ensemble members (Sources/mod_Ensemble.F90, task_SetEns) as
  * independent task_Fall3d copies, one per MPI communicator
