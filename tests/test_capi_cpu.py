@@ -72,4 +72,12 @@ def test_initialize_rejects_grids_past_the_32bit_buffer_limit():
     if not os.path.exists(ads.LIB_PATH):
         pytest.skip("library not built")
     with pytest.raises(ads.Fall3dAdsError, match="2 GiB"):
-        ads.ADS(4096, 4096, 40, 1)  # one bin of c: 4100*4100*44*4 B > 2 GiB
+        ads.ADS(4096, 4096, 40, 1)  # one bin of c: 4128 (padded row) * 4100 * 44 * 4 B > 2 GiB
+
+
+@pytest.mark.parametrize("dims", [(1, 4, 4, 1), (4, 1, 4, 1), (4, 4, 1, 1), (4, 4, 4, 0)])
+def test_initialize_rejects_axes_below_two_cells(dims):
+    if not os.path.exists(ads.LIB_PATH):
+        pytest.skip("library not built")
+    with pytest.raises(ads.Fall3dAdsError, match=">= 2 cells"):
+        ads.ADS(*dims)
