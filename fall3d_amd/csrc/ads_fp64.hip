@@ -102,15 +102,24 @@ template <int DIR, int LIM, int NST>
 __global__ void __launch_bounds__(256) k_sweep64(Sweep64Args a) {
   constexpr int LAG = 2 * (NST - 1);  // faces between stage 1 and the last stage
   const int n = a.n, nx = a.nx, ny = a.ny, nz = a.nz;
-  const long total = a.nlines * a.nb;
+  // Thread order: 64 consecutive lines of one bin per wave, the waves after
+  // it the same lines for the other bins, and the blocks renumbered so that
+  // consecutive ones share an XCD: every bin of a line reads the line's winds
+  // and diffusion close together in time, through one L2.
+  const long cw = 64L * a.nb;  // threads per chunk of 64 lines
+  const long total = (a.nlines + 63) / 64 * cw;
+  const long lb = xcd_block(blockIdx.x, gridDim.x);
   const double s = a.stime, s1 = 1.0 - a.stime, dt = a.dt;
   const bool lerp = DIR == 2 || s != 0.0;  // x/y winds at stime 0 are vel1 as given
   const double *dxp = a.dxp + 1;           // dxp[p], p = -1..n+2
   const double *dxb = a.dxb;               // dxb[p], p = 0..n+2
   double acc[4] = {0.0, 0.0, 0.0, 0.0};    // reference flux beg/end, ledger beg/end
-  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < total; g += (long)gridDim.x * blockDim.x) {
-    const int b = (int)(g / a.nlines);
-    const long L = g - (long)b * a.nlines;
+  for (long g = lb * blockDim.x + threadIdx.x; g < total; g += (long)gridDim.x * blockDim.x) {
+    const long chunk = g / cw;
+    const int r = (int)(g - chunk * cw);
+    const int b = r >> 6;
+    const long L = chunk * 64 + (r & 63);
+    if (L >= a.nlines) continue;  // past the last line (the last chunk's idle lanes)
     int t1, t2;  // transverse indices (x: j, k; y: i, k; z: i, j)
     ptrdiff_t cb, cs;
     size_t vb, vst, kb, sb = 0;
@@ -298,7 +307,7 @@ __global__ void k_source64(double *__restrict__ c, const size_t *__restrict__ id
 }  // namespace
 
 int sweep64_blocks(long nlines, int nb) {
-  const long g = (nlines * (long)nb + 255) / 256;
+  const long g = ((nlines + 63) / 64 * 64 * (long)nb + 255) / 256;
   return (int)(g < 1 ? 1 : (g > kSweep64MaxBlocks ? kSweep64MaxBlocks : g));
 }
 

@@ -28,6 +28,16 @@ namespace f3d {
 __host__ __device__ constexpr int state_pitch(int nx) { return (nx + 4 + 31) & ~31; }
 constexpr int kStateLead = 30;  // floats between the allocation and the state pointer
 
+// Workgroup order: consecutive workgroups are dispatched round-robin over
+// the 8 XCDs (each with its own L2). Renumber so that each XCD receives a
+// contiguous range of logical blocks: neighbouring x tiles and the bin
+// groups of the same columns then share an L2.
+__device__ __forceinline__ int xcd_block(const int b, const int nblocks) {
+  const int q = nblocks >> 3, r = nblocks & 7;
+  const int xcd = b & 7, idx = b >> 3;
+  return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
 struct SweepArgs {
   float *chi;             // state (hi part)
   float *clo;             // low part (compensated state) or nullptr

@@ -698,16 +698,6 @@ __device__ __forceinline__ void stream_body(const SweepArgs &a, const int bid) {
   block_partials5(rb, re, lb, le, rr, a.partials, a.rpartials, bid);
 }
 
-// Workgroup order: consecutive workgroups are dispatched round-robin over
-// the 8 XCDs (each with its own L2). Renumber so that each XCD receives a
-// contiguous range of logical blocks: neighbouring x tiles and the bin
-// groups of the same columns then share an L2.
-__device__ __forceinline__ int xcd_block(const int b, const int nblocks) {
-  const int q = nblocks >> 3, r = nblocks & 7;
-  const int xcd = b & 7, idx = b >> 3;
-  return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
-}
-
 template <int DIR, int LIM, bool COMP, int NST, int BPT, bool FOUT = false>
 __global__ void __launch_bounds__(512) k_stream(SweepArgs a) {
   stream_body<DIR, LIM, COMP, NST, BPT, FOUT>(a, xcd_block(blockIdx.x, gridDim.x));
